@@ -1,0 +1,127 @@
+/*
+ * primesieve_gpu.h -- C ABI of the MI355X (gfx950) HIP counting backend.
+ *
+ * Plain C types only (no C++, no torch): this is what a ctypes / cgo / JNI /
+ * N-API binding, or the reference's own C API, binds.  Every function takes
+ * the backend's global lock, like the reference's GPU API
+ * (src/gpu/GpuSieve.cpp:27,186-343).
+ *
+ * Error convention of the reference C API (include/primesieve.h:28-31,
+ * src/api-c.cpp:143-225): functions returning uint64_t return PSG_ERROR
+ * (~0) and set errno = EDOM on failure, after printing the reason to stderr;
+ * int-returning functions return 0 on success / false and a negative value
+ * on error, with the reason available from psg_gpu_last_error().
+ */
+#ifndef PRIMESIEVE_GPU_H
+#define PRIMESIEVE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSG_ERROR ((uint64_t) ~((uint64_t) 0))
+
+/* ---- device selection and configuration --------------------------------
+ * Replace gpu_available / gpu_devices / set_gpu_device / get_gpu_device /
+ * set_gpu_enabled / get_gpu_enabled and the four tuning knobs
+ * (reference include/primesieve/gpu.hpp:32-56, src/gpu/GpuSieve.cpp:184-254).
+ */
+int  psg_gpu_available(void);
+int  psg_gpu_device_count(void);
+/* Writes DeviceInfo::describe() of device `index` (reference
+ * src/gpu/GpuSieve.cpp:43-55) into buf; returns its length or -1. */
+int  psg_gpu_device_describe(int index, char* buf, size_t len);
+void psg_set_gpu_device(int device_index);
+int  psg_get_gpu_device(void);
+void psg_set_gpu_enabled(int enable);
+int  psg_get_gpu_enabled(void);
+void psg_set_gpu_sieve_size(int kilobytes);
+void psg_set_gpu_work_group_size(int size);
+void psg_set_gpu_segments_per_launch(int segments);
+void psg_set_gpu_presieve(int enable);
+
+/* gpu_active_device / gpu_last_error (reference gpu.hpp:58-63): copy the
+ * string into buf (truncated, NUL-terminated); return its full length. */
+int  psg_gpu_active_device(char* buf, size_t len);
+int  psg_gpu_last_error(char* buf, size_t len);
+
+/* gpu_worthwhile (reference src/gpu/GpuSieve.cpp:272-277): stop - start >= 1e8 */
+int  psg_gpu_worthwhile(uint64_t start, uint64_t stop);
+
+/* ---- counts -------------------------------------------------------------
+ * gpu_count_primes .. gpu_count_sextuplets (reference
+ * src/gpu/GpuSieve.cpp:279-337).  PSG_ERROR + errno = EDOM when no GPU is
+ * usable or the device fails.
+ */
+uint64_t psg_gpu_count_primes(uint64_t start, uint64_t stop);
+uint64_t psg_gpu_count_twins(uint64_t start, uint64_t stop);
+uint64_t psg_gpu_count_triplets(uint64_t start, uint64_t stop);
+uint64_t psg_gpu_count_quadruplets(uint64_t start, uint64_t stop);
+uint64_t psg_gpu_count_quintuplets(uint64_t start, uint64_t stop);
+uint64_t psg_gpu_count_sextuplets(uint64_t start, uint64_t stop);
+
+/* gpu_try_count (reference src/gpu/GpuSieve.cpp:341-365): 1 and *result when
+ * the GPU counted; 0 when it is disabled, unavailable, the range is below
+ * the 1e8 threshold, or the device failed (the caller then sieves on the
+ * CPU).  countType 0..5 = primes..sextuplets.  Never fails loudly. */
+int psg_gpu_try_count(uint64_t start, uint64_t stop, int count_type, uint64_t* result);
+
+/* New: all six counts of [start, stop] from one sieve pass.  0 on success,
+ * -1 on error (see psg_gpu_last_error). */
+int psg_gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6]);
+
+/* ---- measurement --------------------------------------------------------
+ * No reference counterpart.  Statistics of the last count call, measured
+ * with HIP events on the backend's own stream. */
+typedef struct psg_stats {
+  double   kernel_ms;        /* sieve kernels, first launch start to last end */
+  double   fill_ms;          /* big-prime bucket fill kernels */
+  double   wall_ms;          /* host wall time of the call */
+  uint64_t segments;         /* segments sieved */
+  uint64_t bucket_entries;   /* big-prime bucket entries written and read */
+  uint32_t launches;         /* sieve kernel launches */
+  uint32_t sieve_bytes;      /* LDS bytes per segment */
+  uint32_t work_group_size;  /* threads per workgroup */
+  uint32_t sieving_primes;   /* primes >= 29 with p*p <= stop */
+  uint32_t sieve_launch_sum_us; /* sum over sieve launches of their durations (us) */
+  uint32_t reserved;
+} psg_stats;
+int psg_gpu_last_stats(psg_stats* out);
+
+/* Algorithmic work of sieving [start, stop] with segments of seg_bytes:
+ * crossings by sieving primes p >= 29 (all tiers), the 32-bit words of the
+ * bit array, and the crossings by primes above 8*seg_bytes ("big" primes).
+ * Closed form on the host; no device needed. */
+typedef struct psg_work {
+  uint64_t crossings;        /* C: sum over 29 <= p <= isqrt(stop) */
+  uint64_t words32;          /* W = ceil(bytes / 4) */
+  uint64_t crossings_big;    /* crossings by p > 8 * seg_bytes */
+  uint64_t sieving_primes;   /* primes in [29, isqrt(stop)] */
+} psg_work;
+int psg_algorithmic_work(uint64_t start, uint64_t stop, uint32_t seg_bytes, psg_work* out);
+
+/* Measured ceilings of the active device (microbenchmark kernels):
+ * ds_and_b32 throughput on LDS with conflict-free and with random word
+ * addresses (GB/s counted as 8 bytes -- one 4-byte read plus one 4-byte
+ * write -- per atomic, the convention of the sieve's LDS roofline), LDS
+ * ds_read_b64 streaming, and HBM copy (read + write bytes).  0 on success. */
+typedef struct psg_roofline {
+  double and_conflict_free_gbs;
+  double and_random_gbs;
+  double and_conflict_free_gops;   /* 1e9 atomics per second */
+  double and_random_gops;
+  double lds_read_gbs;
+  double hbm_copy_gbs;
+  uint32_t compute_units;
+  uint32_t reserved;
+} psg_roofline;
+int psg_measure_roofline(psg_roofline* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,36 @@
+///
+/// @file  GpuSieve.hpp
+/// @brief Internal hooks of GpuSieve.cpp shared with the C ABI layer.
+///
+#ifndef PSG_GPU_SIEVE_HPP
+#define PSG_GPU_SIEVE_HPP
+
+#include <stdint.h>
+#include <mutex>
+#include <string>
+
+namespace psg { class HipBackend; }
+
+namespace primesieve {
+namespace gpu {
+
+class GpuBackend;
+
+/// The global lock every public GPU entry point holds.
+std::mutex& gpuMutex();
+/// Backend of the selected device or nullptr (caller holds gpuMutex()).
+GpuBackend* activeBackend();
+psg::HipBackend* activeHipBackend();
+/// The backend if one exists, without creating it (caller holds gpuMutex()).
+psg::HipBackend* existingHipBackend();
+/// g_lastError (caller holds gpuMutex()).
+std::string& lastErrorRef();
+/// Drop the backend after a device failure (caller holds gpuMutex()).
+void resetBackend();
+/// Six counts of [start, stop] including 2/3/5; false when no GPU is usable.
+bool countAllOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6]);
+
+} // namespace gpu
+} // namespace primesieve
+
+#endif

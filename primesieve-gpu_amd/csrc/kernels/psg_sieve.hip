@@ -1,0 +1,280 @@
+// psg_sieve.hip -- gfx950 segment kernel of the wheel-30 prime / k-tuplet
+// counter.  Built with `hipcc --genco --offload-arch=gfx950` into a code
+// object that the host library loads with hipModuleLoadData.
+//
+// One workgroup sieves one segment of `segBytes` global bytes held in LDS.
+// Bit b of global byte g stands for 30*g + {7,11,13,17,19,23,29,31}[b]
+// (the reference layout, src/gpu/GpuWheel.hpp:125).  The work of the
+// reference kernel ps_count (src/gpu/cuda/sieve.cu:157-320) is reorganised
+// for CDNA4:
+//
+//   1. fill:   two pre-sieve tiles remove the multiples of 7..23
+//              (periods 7*11*13*17 and 19*23 bytes, read as sliding 32-bit
+//              windows), then the tile primes' own bits are put back;
+//   2. strike: every sieving prime p >= 29 is split into its 8 residue
+//              classes.  In class j the multiples p*q (q coprime to 30) lie at
+//              global bytes c_j + t*p, so a class is an arithmetic progression
+//              with byte stride p and a constant bit -- no wheel state
+//              machine and no per-crossing table reads.  Three tiers:
+//                tier 1 (p <= S/8): one prime per wave; lane = (class, hit),
+//                        stride 8p, so the LDS word mask is loop-invariant;
+//                tier 2 (S/8 < p <= S): lane = (prime, class), stride p with
+//                        a rotating mask;
+//                tier 3 (p > S): lane = prime, at most one hit per class;
+//              plus, optionally, bucket entries produced by psg_bucket_fill
+//              for primes above tier 3 (one LDS atomic per entry);
+//   3. count:  popcount and byte-lane SWAR masks give all six counts in one
+//              pass; wave sums use ds_swizzle butterflies.
+//
+// Strikes are ds_and_b32 atomics on LDS words: several lanes may hit the
+// same word at once.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../kernel_abi.h"
+
+namespace {
+
+__device__ __constant__ uint32_t kOffsets[8] = {7, 11, 13, 17, 19, 23, 29, 31};
+
+// R = B mod p for B < 2^63 using M = ceil(2^(64+L)/p):
+// floor(B/p) = mulhi(B, M) >> L.  Only the low 32 bits of B - q*p matter.
+__device__ __forceinline__ uint32_t mod_magic(uint64_t B, uint32_t p, uint64_t M)
+{
+  const uint32_t L = 31u - (uint32_t) __clz((int) p);
+  const uint64_t q = __umul64hi(B, M) >> L;
+  return (uint32_t) B - (uint32_t) q * p;
+}
+
+// Start (byte index relative to segment base B) of residue class j of prime
+// p, given R = B mod p.  ctab[(p % 30) * 8 + j] packs q_j (the cofactor
+// residue mod 30 of class j) in bits 0..7 and e_j + 1 in bits 8..15, where
+// c_j = (p/30)*q_j + e_j is the global byte of p*q_j.  The result is < p,
+// except that the prime itself (q_j == 1, first period) is skipped.
+__device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t B,
+                                                uint32_t j, const uint16_t* ctab)
+{
+  const uint32_t a = p / 30u;
+  const uint32_t b = p - a * 30u;
+  const uint32_t ent = ctab[b * 8u + j];
+  const uint32_t q = ent & 0xffu;
+  const uint32_t c = a * q + (ent >> 8) - 1u;
+  uint32_t i = (c >= R) ? c - R : c - R + p;
+  if (q == 1u && B + i < (uint64_t) p) {
+    const uint64_t ii = (uint64_t) i + p;
+    i = (ii > 0xffffffffull) ? 0xffffffffu : (uint32_t) ii;
+  }
+  return i;
+}
+
+__device__ __forceinline__ void strike(uint32_t* sieve, uint32_t byteIdx, uint32_t bit)
+{
+  atomicAnd(&sieve[byteIdx >> 2], ~(1u << (((byteIdx & 3u) << 3) + bit)));
+}
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
+{
+  return (x << r) | (x >> (32u - r));
+}
+
+// Sum over the 64 lanes of a wave: xor butterflies inside each 32-lane half
+// with ds_swizzle (bitmask mode: and_mask 0x1f, xor_mask 1..16), then the two
+// halves through readlane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+  v += (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, 0x041f);
+  v += (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, 0x081f);
+  v += (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, 0x101f);
+  v += (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, 0x201f);
+  v += (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, 0x401f);
+  return (uint32_t) __builtin_amdgcn_readlane((int) v, 0) +
+         (uint32_t) __builtin_amdgcn_readlane((int) v, 32);
+}
+
+// Zero the bytes of an 8-byte sieve word outside [gLo, gHi] and apply the
+// partial-byte masks of the range ends.  gw = global byte of the word's byte 0.
+__device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRange& r)
+{
+  if (gw > r.gHi || gw + 7 < r.gLo)
+    return 0;
+  uint64_t out = 0;
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint64_t g = gw + k;
+    uint32_t byte = (uint32_t) (x >> (8 * k)) & 0xffu;
+    if (g < r.gLo || g > r.gHi) byte = 0;
+    if (g == r.gLo) byte &= r.loMask;
+    if (g == r.gHi) byte &= r.hiMask;
+    out |= (uint64_t) byte << (8 * k);
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(1024)
+psg_sieve_count(PsgSieveArgs args)
+{
+  extern __shared__ uint32_t sieve[];
+  __shared__ uint16_t ctab[240];
+  __shared__ uint32_t qhead;
+  __shared__ uint32_t wsum[PSG_NCOUNTS];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthreads = blockDim.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t S = args.segBytes;
+  const uint32_t words = S >> 2;
+  const uint32_t seg = args.segBegin + blockIdx.x;
+  const uint64_t B = args.range.B0 + (uint64_t) seg * S;
+
+  const uint32_t* __restrict__ primes = (const uint32_t*) args.primes;
+  const uint64_t* __restrict__ magics = (const uint64_t*) args.magics;
+  const uint32_t* __restrict__ tile0 = (const uint32_t*) args.tile0;
+  const uint32_t* __restrict__ tile1 = (const uint32_t*) args.tile1;
+
+  // residue-class table, derived from first principles:
+  // for b = p mod 30 and class j, q_j = o_j * b^-1 (mod 30), e_j = (b*q_j - o_j)/30
+  if (tid < 240) {
+    const uint32_t b = tid >> 3, j = tid & 7u;
+    const uint32_t o = kOffsets[j];
+    uint16_t ent = 0;
+    if ((b % 2u) && (b % 3u) && (b % 5u)) {
+      for (uint32_t q = 1; q < 30; q++)
+        if ((b * q) % 30u == o % 30u) {
+          const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
+          ent = (uint16_t) (q | ((uint32_t) (e + 1) << 8));
+          break;
+        }
+    }
+    ctab[tid] = ent;
+  }
+  if (tid == 0) qhead = 0;
+  if (tid < PSG_NCOUNTS) wsum[tid] = 0;
+
+  // ---- 1. fill from the pre-sieve tiles ----------------------------------
+  if (!tile0) {
+    // pre-sieving disabled: every prime >= 7 is struck in step 2
+    for (uint32_t w = tid; w < words; w += nthreads)
+      sieve[w] = 0xffffffffu;
+  } else {
+    uint32_t i0 = (uint32_t) ((B + 4ull * tid) % PSG_TILE0_PERIOD);
+    uint32_t i1 = (uint32_t) ((B + 4ull * tid) % PSG_TILE1_PERIOD);
+    const uint32_t s0 = (4u * nthreads) % PSG_TILE0_PERIOD;
+    const uint32_t s1 = (4u * nthreads) % PSG_TILE1_PERIOD;
+    for (uint32_t w = tid; w < words; w += nthreads) {
+      sieve[w] = tile0[i0] & tile1[i1];
+      i0 += s0; if (i0 >= PSG_TILE0_PERIOD) i0 -= PSG_TILE0_PERIOD;
+      i1 += s1; if (i1 >= PSG_TILE1_PERIOD) i1 -= PSG_TILE1_PERIOD;
+    }
+    // the tiles also cleared 7..23 themselves: global byte 0 is written by
+    // thread 0 above, so thread 0 restores it without a barrier
+    if (B == 0 && tid == 0)
+      sieve[0] |= 0x3fu;
+  }
+  __syncthreads();
+
+  // ---- 2. strike the sieving primes --------------------------------------
+  {
+    const uint32_t np = args.range.nprimes;
+    const uint32_t t1 = min(args.tier1End, np);
+    const uint32_t t2 = min(args.tier2End, np);
+    const uint32_t t3 = min(args.tier3End, np);
+    const uint32_t n1 = t1;
+    const uint32_t n2 = (t2 - t1 + 7u) >> 3;
+    const uint32_t n3 = (t3 - t2 + 63u) >> 6;
+    const uint32_t total = n1 + n2 + n3;
+
+    for (;;) {
+      uint32_t item = 0;
+      if (lane == 0) item = atomicAdd(&qhead, 1u);
+      item = (uint32_t) __builtin_amdgcn_readfirstlane((int) item);
+      if (item >= total) break;
+
+      if (item < n1) {
+        // tier 1: whole wave on one prime; lane = class (lane & 7), hit (lane >> 3)
+        const uint32_t p = primes[item];
+        const uint32_t R = mod_magic(B, p, magics[item]);
+        const uint32_t j = lane & 7u;
+        const uint32_t i = class_start(p, R, B, j, ctab);
+        const uint32_t pos = i + (lane >> 3) * p;
+        const uint32_t mask = ~(1u << (((pos & 3u) << 3) + j));
+        const uint32_t step = 8u * p;
+        for (uint32_t addr = pos & ~3u; addr < S; addr += step)
+          atomicAnd(&sieve[addr >> 2], mask);
+      } else if (item < n1 + n2) {
+        // tier 2: lane = (prime slot, class), stride p, mask rotates by 8*(p mod 4)
+        const uint32_t k = t1 + ((item - n1) << 3) + (lane >> 3);
+        if (k < t2) {
+          const uint32_t p = primes[k];
+          const uint32_t R = mod_magic(B, p, magics[k]);
+          const uint32_t j = lane & 7u;
+          uint32_t i = class_start(p, R, B, j, ctab);
+          uint32_t mask = ~(1u << (((i & 3u) << 3) + j));
+          const uint32_t rot = (p & 3u) << 3;
+          for (; i < S; i += p) {
+            atomicAnd(&sieve[i >> 2], mask);
+            mask = rotl32(mask, rot);
+          }
+        }
+      } else {
+        // tier 3: lane = prime, at most one hit per class
+        const uint32_t k = t2 + ((item - n1 - n2) << 6) + lane;
+        if (k < t3) {
+          const uint32_t p = primes[k];
+          const uint32_t R = mod_magic(B, p, magics[k]);
+          for (uint32_t j = 0; j < 8; j++) {
+            const uint32_t i = class_start(p, R, B, j, ctab);
+            if (i < S) strike(sieve, i, j);
+          }
+        }
+      }
+    }
+  }
+
+  // ---- 2b. big-prime bucket entries for this segment ---------------------
+  if (args.buckets) {
+    const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
+    const uint32_t* __restrict__ offs = (const uint32_t*) args.bucketOffs;
+    const uint32_t ls = seg - args.slabSegBegin;
+    const uint32_t e0 = offs[ls], e1 = offs[ls + 1];
+    // entry = (byte index << 3) | class
+    for (uint32_t e = e0 + tid; e < e1; e += nthreads) {
+      const uint32_t v = ent[e];
+      strike(sieve, v >> 3, v & 7u);
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. count all six types --------------------------------------------
+  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+  {
+    const uint64_t* s64 = (const uint64_t*) sieve;
+    const uint32_t words64 = S >> 3;
+    for (uint32_t w = tid; w < words64; w += nthreads) {
+      uint64_t x = s64[w];
+      const uint64_t gw = B + 8ull * w;
+      if (gw <= args.range.gLo || gw + 7 >= args.range.gHi)
+        x = mask_edges(x, gw, args.range);
+      const uint64_t t1 = x & (x >> 1);
+      const uint64_t t2 = t1 & (x >> 2);
+      const uint64_t t3 = t2 & (x >> 3);
+      const uint64_t t4 = t3 & (x >> 4);
+      const uint64_t t5 = t4 & (x >> 5);
+      c0 += __popcll(x);
+      c1 += __popcll(t1 & 0x4a4a4a4a4a4a4a4aull);   // (11,13) (17,19) (29,31)
+      c2 += __popcll(t2 & 0x0f0f0f0f0f0f0f0full);   // 3 consecutive bits from 0..3
+      c3 += __popcll(t3 & 0x0202020202020202ull);   // bits 1..4
+      c4 += __popcll(t4 & 0x0303030303030303ull);   // bits 0..4, 1..5
+      c5 += __popcll(t5 & 0x0101010101010101ull);   // bits 0..5
+    }
+  }
+  c0 = wave_sum(c0); c1 = wave_sum(c1); c2 = wave_sum(c2);
+  c3 = wave_sum(c3); c4 = wave_sum(c4); c5 = wave_sum(c5);
+  if (lane == 0) {
+    atomicAdd(&wsum[0], c0); atomicAdd(&wsum[1], c1); atomicAdd(&wsum[2], c2);
+    atomicAdd(&wsum[3], c3); atomicAdd(&wsum[4], c4); atomicAdd(&wsum[5], c5);
+  }
+  __syncthreads();
+  if (tid < PSG_NCOUNTS && wsum[tid])
+    atomicAdd((unsigned long long*) args.counts + tid, (unsigned long long) wsum[tid]);
+}

@@ -1,0 +1,144 @@
+"""GPU parity: the gfx950 kernels, called through the C ABI, against the
+reference's golden vectors and the CPU oracle.  Every count is compared
+bit-exact.  Mirrors the reference GPU parity test (test/gpu_count.cpp:73-132):
+its whole case list, the known pi(x) values, plus the configs of the
+benchmark and seeded random ranges checked against the oracle."""
+import random
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_device_and_geometry(gpu):
+    s = gpu.devices()
+    assert s and "HIP" in s[0]
+    gpu.count_all(0, 10 ** 6)
+    st = gpu.last_stats()
+    assert st.sieve_bytes % 256 == 0 and st.work_group_size % 64 == 0
+    _say("stats", st.as_dict())
+
+
+def test_reference_case_list_per_type(gpu, golden):
+    """test/gpu_count.cpp:86-113 through the per-type entry points
+    (gpu_count_primes .. gpu_count_sextuplets)."""
+    cases = [c for c in golden("cases.json") if "tag" not in c]
+    bad = []
+    for c in cases:
+        for t, want in c["counts"].items():
+            got = gpu.count(c["start"], c["stop"], int(t))
+            if got != want:
+                bad.append((c["start"], c["stop"], t, got, want))
+    assert not bad, bad[:20]
+
+
+def test_reference_case_list_single_pass(gpu, golden):
+    """Same cases through the single-pass six-count call."""
+    bad = []
+    for c in golden("cases.json"):
+        if c.get("tag") == "config":
+            continue
+        got = gpu.count_all(c["start"], c["stop"])
+        for t, want in c["counts"].items():
+            if got[int(t)] != want:
+                bad.append((c["start"], c["stop"], t, got[int(t)], want))
+    assert not bad, bad[:20]
+
+
+def test_known_pi_and_kats(gpu, golden):
+    """Known values from the reference test suite (kats.json); ranges the
+    per-segment tiers finish quickly."""
+    bad = []
+    for e in golden("kats.json"):
+        if e["start"] >= 10 ** 18:
+            continue   # needs the big-prime bucket tier (tested separately)
+        _say("kat", e["start"], e["stop"], e["type"])
+        got = gpu.count(e["start"], e["stop"], e["type"])
+        if got != e["count"]:
+            bad.append((e, got))
+    assert not bad, bad
+
+
+def test_config1_pi_1e10(gpu, golden):
+    want = [c for c in golden("cases.json") if c.get("tag") == "config"][0]
+    got = gpu.count_all(want["start"], want["stop"])
+    assert got[0] == 455052511
+    assert got == [want["counts"][str(t)] for t in range(6)]
+
+
+def test_config2_six_counts(gpu, golden):
+    """[1e13, 1e13+1e11]: six counts from one pass, against the reference."""
+    want = [c for c in golden("cases.json") if c.get("tag") == "config"][1]
+    got = gpu.count_all(want["start"], want["stop"])
+    _say("config2", got, gpu.last_stats().as_dict())
+    assert got == [want["counts"][str(t)] for t in range(6)]
+    assert got[0] == 3340141707
+
+
+def test_random_ranges_golden(gpu, golden):
+    bad = []
+    for c in golden("cases.json"):
+        if c.get("tag") != "random":
+            continue
+        got = gpu.count_all(c["start"], c["stop"])
+        if got != [c["counts"][str(t)] for t in range(6)]:
+            bad.append((c["start"], c["stop"], got, c["counts"]))
+    assert not bad, bad[:5]
+
+
+@pytest.mark.parametrize("kb,wg", [(16, 256), (32, 512), (64, 1024), (128, 512), (150, 256)])
+def test_geometries_against_oracle(gpu, oracle, kb, wg):
+    """Every segment size / workgroup size gives the oracle's counts,
+    including ranges that straddle segment ends and byte ends."""
+    gpu.set_sieve_size(kb)
+    gpu.set_work_group_size(wg)
+    try:
+        rng = random.Random(kb * 1000 + wg)
+        for base in [0, 7, 10 ** 6, 10 ** 9, 2 ** 32 + 1, 10 ** 12, 10 ** 14]:
+            for _ in range(4):
+                a = base + rng.randrange(0, 10 ** 6)
+                b = a + rng.choice([0, 1, 29, 30, 31, rng.randrange(10 ** 3),
+                                    rng.randrange(10 ** 7), rng.randrange(10 ** 8)])
+                got = gpu.count_all(a, b)
+                want = oracle.count6(a, b)
+                assert got == want, (kb, wg, a, b, got, want)
+    finally:
+        gpu.set_sieve_size(0)
+        gpu.set_work_group_size(0)
+
+
+def test_presieve_disabled(gpu, oracle):
+    gpu.set_presieve(False)
+    try:
+        for a, b in [(0, 10 ** 7), (10 ** 9, 10 ** 9 + 10 ** 7), (0, 100), (7, 7)]:
+            assert gpu.count_all(a, b) == oracle.count6(a, b), (a, b)
+    finally:
+        gpu.set_presieve(True)
+
+
+def test_empty_and_tiny_ranges(gpu):
+    # expected values computed by the reference CPU library (oracle/_ref)
+    assert gpu.count_all(10, 9) == [0] * 6
+    assert gpu.count_all(0, 0) == [0] * 6
+    assert gpu.count_all(2, 2) == [1, 0, 0, 0, 0, 0]
+    assert gpu.count_all(5, 17) == [5, 2, 3, 1, 1, 0]
+    assert gpu.count_all(0, 17) == [7, 3, 3, 1, 1, 0]
+    assert gpu.count_all(0, 30) == [10, 4, 5, 2, 3, 1]
+    assert gpu.count_all(3, 31) == [10, 5, 5, 2, 3, 1]
+    assert gpu.count(0, 17, 4) == 1
+
+
+def test_try_count_policy(gpu):
+    gpu.set_enabled(False)
+    assert gpu.try_count(0, 10 ** 9) is None          # disabled
+    gpu.set_enabled(True)
+    try:
+        assert gpu.try_count(0, 10 ** 7) is None      # below the 1e8 threshold
+        assert gpu.try_count(0, 10 ** 9) == 50847534
+        assert gpu.try_count(0, 10 ** 9, 1) == 3424506
+    finally:
+        gpu.set_enabled(False)
