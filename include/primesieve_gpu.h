@@ -77,8 +77,9 @@ int psg_gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6]);
  * No reference counterpart.  Statistics of the last count call, measured
  * with HIP events on the backend's own stream. */
 typedef struct psg_stats {
-  double   kernel_ms;        /* sieve kernels, first launch start to last end */
-  double   fill_ms;          /* big-prime bucket fill kernels */
+  double   kernel_ms;        /* sum of the segment kernel launch durations */
+  double   fill_ms;          /* sum of the big-prime bucket kernel durations */
+  double   device_ms;        /* first device event to last */
   double   wall_ms;          /* host wall time of the call */
   uint64_t segments;         /* segments sieved */
   uint64_t bucket_entries;   /* big-prime bucket entries written and read */
@@ -86,8 +87,6 @@ typedef struct psg_stats {
   uint32_t sieve_bytes;      /* LDS bytes per segment */
   uint32_t work_group_size;  /* threads per workgroup */
   uint32_t sieving_primes;   /* primes >= 29 with p*p <= stop */
-  uint32_t sieve_launch_sum_us; /* sum over sieve launches of their durations (us) */
-  uint32_t reserved;
 } psg_stats;
 int psg_gpu_last_stats(psg_stats* out);
 

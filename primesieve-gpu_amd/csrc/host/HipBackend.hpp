@@ -72,6 +72,7 @@ private:
   void init();
   void ensureSievingPrimes(uint32_t limit);
   void launchSieve(const PsgSieveArgs& args, uint32_t nseg);
+  void recordPhase(size_t i);
 
   const HipApi* api_;
   primesieve::gpu::DeviceInfo dev_;
@@ -79,9 +80,17 @@ private:
   uint32_t sieveBytes_ = 0;
   uint32_t wgSize_ = 0;
   uint32_t segmentsPerLaunch_ = 0;
+  uint32_t bucketFactor_ = 1;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
+  uint32_t slabSegments_ = 0;     // segments per bucket slab
+  uint32_t bucketChunks_ = 0;     // target workgroups of the bucket count / fill kernels
+  uint32_t windowSegs_ = 64;      // segments per bucket window
 
   hipModule_t module_ = nullptr;
   hipFunction_t fnSieve_ = nullptr;
+  hipFunction_t fnBucketCount_ = nullptr;
+  hipFunction_t fnScanChunks_ = nullptr;
+  hipFunction_t fnScanLists_ = nullptr;
+  hipFunction_t fnBucketFill_ = nullptr;
   hipStream_t stream_ = nullptr;
   hipEvent_t evStart_ = nullptr;
   hipEvent_t evStop_ = nullptr;
@@ -92,8 +101,15 @@ private:
   DeviceBuffer tile1_;
   DeviceBuffer counts_;      // uint64[6]
   uint64_t* hostCounts_ = nullptr;   // pinned
+  DeviceBuffer bucketCounts_;        // uint32 [chunks][lists]
+  DeviceBuffer listOff_;             // uint32 [lists + 1]
+  DeviceBuffer entries_;             // bucket entries
+  DeviceBuffer overflow_;            // uint32 flag
+  uint32_t* hostOverflow_ = nullptr; // pinned
+  std::vector<hipEvent_t> phaseEvents_;
 
   std::vector<uint32_t> hostPrimes_; // the primes held in primes_
+  std::vector<double> invPrefix_;    // invPrefix_[i] = sum_{j<i} 1/p_j
   uint32_t primesLimit_ = 0;         // every prime <= primesLimit_ is held
   uint32_t firstCrossPrime_ = 29;
 

@@ -47,6 +47,41 @@ typedef struct PsgSieveArgs {
   uint32_t tier2End;    /* primes[tier1End, tier2End): p <= segBytes */
   uint32_t tier3End;    /* primes[tier2End, tier3End): per-segment big primes */
   uint32_t slabSegBegin;/* first segment index covered by the bucket slab */
+  uint32_t entryBytes;  /* 2: uint16 byte offsets (segBytes <= 65536); 4: uint32 */
+  uint32_t capacity;    /* bucket entries allocated */
+  uint32_t pad1;
 } PsgSieveArgs;
+
+/* Big-prime bucket tier.  A slab is `slabBytes / segBytes` consecutive
+ * segments starting at global byte slabB.  Every prime in
+ * primes[bigBegin, bigEnd) is walked over the whole slab once per phase;
+ * its hits go to list (segment * 8 + class), class = bit index.  Lists are
+ * laid out by a counting sort with no global atomics:
+ *   psg_bucket_count  per-workgroup LDS histogram -> counts[chunk][list]
+ *   psg_bucket_scan_chunks  exclusive prefix over chunks per list, totals
+ *   psg_bucket_scan_lists   exclusive prefix over lists -> listOff
+ *   psg_bucket_fill   re-walk, LDS cursors, store byte offsets
+ * The sieve kernel then strikes list (s*8 + j) of segment s with bit j. */
+typedef struct PsgBucketArgs {
+  uint64_t slabB;       /* global byte index of the slab's first byte */
+  uint64_t primes;      /* const uint32_t* */
+  uint64_t magics;      /* const uint64_t* */
+  uint64_t counts;      /* uint32_t[nChunks][nLists] */
+  uint64_t listOff;     /* uint32_t[nLists + 1] */
+  uint64_t entries;     /* uint16_t* or uint32_t* (entryBytes) */
+  uint64_t overflow;    /* uint32_t flag, set when capacity is exceeded */
+  uint32_t slabBytes;   /* bytes covered, multiple of segBytes, < 2^31 */
+  uint32_t segShift;    /* log2(segBytes) */
+  uint32_t nLists;      /* 8 * segments in the slab */
+  uint32_t bigBegin;
+  uint32_t bigEnd;
+  uint32_t nChunks;     /* prime chunks (primes interleaved over chunks) */
+  uint32_t capacity;    /* entries allocated */
+  uint32_t entryBytes;  /* 2 or 4 */
+  uint32_t windowSegs;  /* segments per window: workgroup (w, c) walks the
+                           primes of chunk c over window w only, so that its
+                           sub-lists are contiguous and its LDS counters few */
+  uint32_t nWindows;    /* ceil(segments in slab / windowSegs) */
+} PsgBucketArgs;
 
 #endif

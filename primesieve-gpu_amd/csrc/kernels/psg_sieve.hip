@@ -109,6 +109,81 @@ __device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRange& r)
   return out;
 }
 
+
+// Residue-class table, derived from first principles: for b = p mod 30 and
+// class j (offset o_j), q_j = o_j * b^-1 (mod 30) and e_j = (b*q_j - o_j)/30,
+// packed as q_j | (e_j + 1) << 8 (e_j is -1 only for o_j = 31, b = 1).
+__device__ void init_ctab(uint16_t* ctab)
+{
+  for (uint32_t t = threadIdx.x; t < 240; t += blockDim.x) {
+    const uint32_t b = t >> 3, j = t & 7u;
+    const uint32_t o = kOffsets[j];
+    uint16_t ent = 0;
+    if ((b % 2u) && (b % 3u) && (b % 5u)) {
+      for (uint32_t q = 1; q < 30; q++)
+        if ((b * q) % 30u == o % 30u) {
+          const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
+          ent = (uint16_t) (q | ((uint32_t) (e + 1) << 8));
+          break;
+        }
+    }
+    ctab[t] = ent;
+  }
+}
+
+// Workgroup (w, c) of the count / fill kernels: window w of the slab
+// (windowSegs segments), prime chunk c.  Primes are interleaved over the
+// chunks (k = bigBegin + c + i * nChunks) so that small and large big primes
+// are spread evenly.
+struct BucketWindow
+{
+  uint32_t w, c;
+  uint32_t firstList;   // global list index of the window's list 0
+  uint32_t nLists;      // lists of this window (8 * its segments)
+  uint32_t bytes;       // bytes of this window
+  uint64_t base;        // global byte of the window's first byte
+};
+
+__device__ __forceinline__ BucketWindow bucket_window(const PsgBucketArgs& a)
+{
+  BucketWindow bw;
+  bw.w = blockIdx.x % a.nWindows;
+  bw.c = blockIdx.x / a.nWindows;
+  const uint32_t seg0 = bw.w * a.windowSegs;
+  const uint32_t segs = min(a.windowSegs, (a.slabBytes >> a.segShift) - seg0);
+  bw.firstList = seg0 * 8u;
+  bw.nLists = segs * 8u;
+  bw.bytes = segs << a.segShift;
+  bw.base = a.slabB + ((uint64_t) seg0 << a.segShift);
+  return bw;
+}
+
+// Walk every hit of the chunk's primes over the window; call
+// emit(windowList, byteOffsetInSegment) per hit.
+template <typename Emit>
+__device__ __forceinline__ void bucket_walk(const PsgBucketArgs& a, const BucketWindow& bw,
+                                            const uint16_t* ctab, Emit emit)
+{
+  const uint32_t* __restrict__ primes = (const uint32_t*) a.primes;
+  const uint64_t* __restrict__ magics = (const uint64_t*) a.magics;
+  const uint32_t segMask = (1u << a.segShift) - 1u;
+  const uint32_t stride = blockDim.x * a.nChunks;
+  const uint32_t bytes = bw.bytes;
+  for (uint32_t k = a.bigBegin + bw.c + threadIdx.x * a.nChunks; k < a.bigEnd; k += stride) {
+    const uint32_t p = primes[k];
+    const uint32_t R = mod_magic(bw.base, p, magics[k]);
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t i = class_start(p, R, bw.base, j, ctab);
+      if (p >= 0x80000000u) {
+        if (i < bytes) emit(((i >> a.segShift) << 3) + j, i & segMask);
+      } else {
+        for (uint32_t pos = i; pos < bytes; pos += p)
+          emit(((pos >> a.segShift) << 3) + j, pos & segMask);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" __global__ void __launch_bounds__(1024)
@@ -132,22 +207,7 @@ psg_sieve_count(PsgSieveArgs args)
   const uint32_t* __restrict__ tile0 = (const uint32_t*) args.tile0;
   const uint32_t* __restrict__ tile1 = (const uint32_t*) args.tile1;
 
-  // residue-class table, derived from first principles:
-  // for b = p mod 30 and class j, q_j = o_j * b^-1 (mod 30), e_j = (b*q_j - o_j)/30
-  if (tid < 240) {
-    const uint32_t b = tid >> 3, j = tid & 7u;
-    const uint32_t o = kOffsets[j];
-    uint16_t ent = 0;
-    if ((b % 2u) && (b % 3u) && (b % 5u)) {
-      for (uint32_t q = 1; q < 30; q++)
-        if ((b * q) % 30u == o % 30u) {
-          const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
-          ent = (uint16_t) (q | ((uint32_t) (e + 1) << 8));
-          break;
-        }
-    }
-    ctab[tid] = ent;
-  }
+  init_ctab(ctab);
   if (tid == 0) qhead = 0;
   if (tid < PSG_NCOUNTS) wsum[tid] = 0;
 
@@ -231,16 +291,22 @@ psg_sieve_count(PsgSieveArgs args)
     }
   }
 
-  // ---- 2b. big-prime bucket entries for this segment ---------------------
+  // ---- 2b. big-prime bucket lists of this segment -----------------------
   if (args.buckets) {
-    const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
     const uint32_t* __restrict__ offs = (const uint32_t*) args.bucketOffs;
     const uint32_t ls = seg - args.slabSegBegin;
-    const uint32_t e0 = offs[ls], e1 = offs[ls + 1];
-    // entry = (byte index << 3) | class
-    for (uint32_t e = e0 + tid; e < e1; e += nthreads) {
-      const uint32_t v = ent[e];
-      strike(sieve, v >> 3, v & 7u);
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t e0 = min(offs[ls * 8u + j], args.capacity);
+      const uint32_t e1 = min(offs[ls * 8u + j + 1u], args.capacity);
+      if (args.entryBytes == 2) {
+        const uint16_t* __restrict__ ent = (const uint16_t*) args.buckets;
+        for (uint32_t e = e0 + tid; e < e1; e += nthreads)
+          strike(sieve, ent[e], j);
+      } else {
+        const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
+        for (uint32_t e = e0 + tid; e < e1; e += nthreads)
+          strike(sieve, ent[e], j);
+      }
     }
   }
   __syncthreads();
@@ -277,4 +343,101 @@ psg_sieve_count(PsgSieveArgs args)
   __syncthreads();
   if (tid < PSG_NCOUNTS && wsum[tid])
     atomicAdd((unsigned long long*) args.counts + tid, (unsigned long long) wsum[tid]);
+}
+
+// ---------------------------------------------------------------------------
+// Big-prime bucket tier (see PsgBucketArgs): counting sort of the slab's hits
+// into per-(segment, class) lists.
+// ---------------------------------------------------------------------------
+
+extern "C" __global__ void __launch_bounds__(512)
+psg_bucket_count(PsgBucketArgs a)
+{
+  extern __shared__ uint32_t hist[];
+  __shared__ uint16_t ctab[240];
+  init_ctab(ctab);
+  const BucketWindow bw = bucket_window(a);
+  for (uint32_t l = threadIdx.x; l < bw.nLists; l += blockDim.x) hist[l] = 0;
+  __syncthreads();
+  bucket_walk(a, bw, ctab, [&](uint32_t list, uint32_t) { atomicAdd(&hist[list], 1u); });
+  __syncthreads();
+  uint32_t* counts = (uint32_t*) a.counts + (uint64_t) bw.c * a.nLists + bw.firstList;
+  for (uint32_t l = threadIdx.x; l < bw.nLists; l += blockDim.x) counts[l] = hist[l];
+}
+
+// counts[c][l] <- sum_{c' < c} counts[c'][l];  listOff[l] <- sum_c counts[c][l]
+extern "C" __global__ void __launch_bounds__(256)
+psg_bucket_scan_chunks(PsgBucketArgs a)
+{
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= a.nLists) return;
+  uint32_t* counts = (uint32_t*) a.counts;
+  uint32_t run = 0;
+  for (uint32_t c = 0; c < a.nChunks; c++) {
+    const uint64_t idx = (uint64_t) c * a.nLists + l;
+    const uint32_t v = counts[idx];
+    counts[idx] = run;
+    run += v;
+  }
+  ((uint32_t*) a.listOff)[l] = run;
+}
+
+// listOff[0..nLists] <- exclusive prefix sum of the list totals (one block)
+extern "C" __global__ void __launch_bounds__(1024)
+psg_bucket_scan_lists(PsgBucketArgs a)
+{
+  __shared__ uint32_t part[1024];
+  uint32_t* off = (uint32_t*) a.listOff;
+  const uint32_t n = a.nLists;
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t b = threadIdx.x * per;
+  const uint32_t e = min(b + per, n);
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; i++) sum += off[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the partial sums
+  for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+    const uint32_t v = (threadIdx.x >= d) ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (uint32_t i = b; i < e; i++) {
+    const uint32_t v = off[i];
+    off[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    off[n] = part[threadIdx.x];
+    if (part[threadIdx.x] > a.capacity) *(uint32_t*) a.overflow = 1u;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(512)
+psg_bucket_fill(PsgBucketArgs a)
+{
+  extern __shared__ uint32_t cursor[];
+  __shared__ uint16_t ctab[240];
+  init_ctab(ctab);
+  const BucketWindow bw = bucket_window(a);
+  const uint32_t* counts = (const uint32_t*) a.counts + (uint64_t) bw.c * a.nLists + bw.firstList;
+  const uint32_t* listOff = (const uint32_t*) a.listOff + bw.firstList;
+  for (uint32_t l = threadIdx.x; l < bw.nLists; l += blockDim.x) cursor[l] = listOff[l] + counts[l];
+  __syncthreads();
+  const uint32_t cap = a.capacity;
+  if (a.entryBytes == 2) {
+    uint16_t* ent = (uint16_t*) a.entries;
+    bucket_walk(a, bw, ctab, [&](uint32_t list, uint32_t off) {
+      const uint32_t slot = atomicAdd(&cursor[list], 1u);
+      if (slot < cap) ent[slot] = (uint16_t) off;
+    });
+  } else {
+    uint32_t* ent = (uint32_t*) a.entries;
+    bucket_walk(a, bw, ctab, [&](uint32_t list, uint32_t off) {
+      const uint32_t slot = atomicAdd(&cursor[list], 1u);
+      if (slot < cap) ent[slot] = off;
+    });
+  }
 }

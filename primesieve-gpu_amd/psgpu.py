@@ -29,6 +29,7 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("kernel_ms", ctypes.c_double),
         ("fill_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
         ("wall_ms", ctypes.c_double),
         ("segments", ctypes.c_uint64),
         ("bucket_entries", ctypes.c_uint64),
@@ -36,12 +37,10 @@ class Stats(ctypes.Structure):
         ("sieve_bytes", ctypes.c_uint32),
         ("work_group_size", ctypes.c_uint32),
         ("sieving_primes", ctypes.c_uint32),
-        ("sieve_launch_sum_us", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Work(ctypes.Structure):
