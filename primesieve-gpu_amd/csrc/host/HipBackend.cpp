@@ -219,6 +219,7 @@ void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
   args.tile1 = tile1_.addr();
   args.counts = counts_.addr();
   args.segBytes = S;
+  args.tier0End = std::min(upto(std::min<uint32_t>(256, S / 8)), nprimes);
   args.tier1End = std::min(upto(S / 8), nprimes);
   args.tier2End = std::min(upto(S), nprimes);
   args.tier3End = nprimes;

@@ -43,13 +43,13 @@ typedef struct PsgSieveArgs {
   uint64_t bucketOffs;  /* const uint32_t*: per-segment entry offsets within the slab */
   uint32_t segBytes;    /* multiple of 256, <= PSG_MAX_SEG_BYTES */
   uint32_t segBegin;    /* first segment index of this launch */
-  uint32_t tier1End;    /* primes[0, tier1End): p <= segBytes/8 */
+  uint32_t tier1End;    /* primes[tier0End, tier1End): p <= segBytes/8 */
   uint32_t tier2End;    /* primes[tier1End, tier2End): p <= segBytes */
   uint32_t tier3End;    /* primes[tier2End, tier3End): per-segment big primes */
   uint32_t slabSegBegin;/* first segment index covered by the bucket slab */
   uint32_t entryBytes;  /* 2: uint16 byte offsets (segBytes <= 65536); 4: uint32 */
   uint32_t capacity;    /* bucket entries allocated */
-  uint32_t pad1;
+  uint32_t tier0End;    /* primes[0, tier0End): p <= 256, struck per wave slice */
 } PsgSieveArgs;
 
 /* Big-prime bucket tier.  A slab is `slabBytes / segBytes` consecutive

@@ -15,14 +15,17 @@
 //              classes.  In class j the multiples p*q (q coprime to 30) lie at
 //              global bytes c_j + t*p, so a class is an arithmetic progression
 //              with byte stride p and a constant bit -- no wheel state
-//              machine and no per-crossing table reads.  Three tiers:
-//                tier 1 (p <= S/8): one prime per wave; lane = (class, hit),
-//                        stride 8p, so the LDS word mask is loop-invariant;
+//              machine and no per-crossing table reads.  Tiers:
+//                tier 0 (p <= 256): each wave strikes all of them over its
+//                        own slice of the segment (even load);
+//                tier 1 (p <= S/8): batches of 8 primes per wave; lane =
+//                        (class, hit), stride 8p, loop-invariant word mask;
 //                tier 2 (S/8 < p <= S): lane = (prime, class), stride p with
 //                        a rotating mask;
-//                tier 3 (p > S): lane = prime, at most one hit per class;
-//              plus, optionally, bucket entries produced by psg_bucket_fill
-//              for primes above tier 3 (one LDS atomic per entry);
+//                tier 3 (S < p <= bucket threshold): lane = prime, at most
+//                        one hit per class;
+//              plus the bucket lists of psg_bucket_fill for larger primes
+//              (one LDS atomic per entry);
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
 //              pass; wave sums use ds_swizzle butterflies.
 //
@@ -191,7 +194,6 @@ psg_sieve_count(PsgSieveArgs args)
 {
   extern __shared__ uint32_t sieve[];
   __shared__ uint16_t ctab[240];
-  __shared__ uint32_t qhead;
   __shared__ uint32_t wsum[PSG_NCOUNTS];
 
   const uint32_t tid = threadIdx.x;
@@ -208,7 +210,6 @@ psg_sieve_count(PsgSieveArgs args)
   const uint32_t* __restrict__ tile1 = (const uint32_t*) args.tile1;
 
   init_ctab(ctab);
-  if (tid == 0) qhead = 0;
   if (tid < PSG_NCOUNTS) wsum[tid] = 0;
 
   // ---- 1. fill from the pre-sieve tiles ----------------------------------
@@ -234,41 +235,89 @@ psg_sieve_count(PsgSieveArgs args)
   __syncthreads();
 
   // ---- 2. strike the sieving primes --------------------------------------
+  // Work is assigned statically (wave w takes batches w, w + nwaves, ...)
+  // and the next batch's primes are loaded before the current batch is
+  // struck, so that global-load latency overlaps LDS work.
   {
     const uint32_t np = args.range.nprimes;
-    const uint32_t t1 = min(args.tier1End, np);
-    const uint32_t t2 = min(args.tier2End, np);
-    const uint32_t t3 = min(args.tier3End, np);
-    const uint32_t n1 = t1;
-    const uint32_t n2 = (t2 - t1 + 7u) >> 3;
-    const uint32_t n3 = (t3 - t2 + 63u) >> 6;
-    const uint32_t total = n1 + n2 + n3;
+    const uint32_t t0 = min(args.tier0End, np);
+    const uint32_t t1 = max(t0, min(args.tier1End, np));
+    const uint32_t t2 = max(t1, min(args.tier2End, np));
+    const uint32_t t3 = max(t2, min(args.tier3End, np));
+    const uint32_t wave = tid >> 6;
+    const uint32_t nwaves = nthreads >> 6;
+    const uint32_t j = lane & 7u;
+    const uint32_t slot = lane >> 3;
 
-    for (;;) {
-      uint32_t item = 0;
-      if (lane == 0) item = atomicAdd(&qhead, 1u);
-      item = (uint32_t) __builtin_amdgcn_readfirstlane((int) item);
-      if (item >= total) break;
+    // tier 0 (p <= 256): every wave strikes all of them over its own slice
+    // of the segment, so the heaviest primes are shared evenly.
+    {
+      const uint32_t sliceBytes = S / nwaves;          // S is a multiple of 256
+      const uint32_t sliceLo = wave * sliceBytes;
+      const uint32_t sliceHi = sliceLo + sliceBytes;
+      const uint64_t Bs = B + sliceLo;
+      for (uint32_t b = 0; b < t0; b += 8) {
+        const uint32_t k = b + slot;
+        const uint32_t p = (k < t0) ? primes[k] : 0u;
+        const uint32_t st = p ? class_start(p, mod_magic(Bs, p, magics[k]), Bs, j, ctab) : 0u;
+        for (uint32_t q = 0; q < 8; q++) {
+          const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
+          if (!ps) break;
+          const uint32_t pos = sliceLo + (uint32_t) __shfl((int) st, (int) (q * 8u + j)) + slot * ps;
+          const uint32_t mask = ~(1u << (((pos & 3u) << 3) + j));
+          const uint32_t step = 8u * ps;
+          for (uint32_t addr = pos & ~3u; addr < sliceHi; addr += step)
+            atomicAnd(&sieve[addr >> 2], mask);
+        }
+      }
+    }
 
-      if (item < n1) {
-        // tier 1: whole wave on one prime; lane = class (lane & 7), hit (lane >> 3)
-        const uint32_t p = primes[item];
-        const uint32_t R = mod_magic(B, p, magics[item]);
-        const uint32_t j = lane & 7u;
-        const uint32_t i = class_start(p, R, B, j, ctab);
-        const uint32_t pos = i + (lane >> 3) * p;
-        const uint32_t mask = ~(1u << (((pos & 3u) << 3) + j));
-        const uint32_t step = 8u * p;
-        for (uint32_t addr = pos & ~3u; addr < S; addr += step)
-          atomicAnd(&sieve[addr >> 2], mask);
-      } else if (item < n1 + n2) {
-        // tier 2: lane = (prime slot, class), stride p, mask rotates by 8*(p mod 4)
-        const uint32_t k = t1 + ((item - n1) << 3) + (lane >> 3);
-        if (k < t2) {
-          const uint32_t p = primes[k];
-          const uint32_t R = mod_magic(B, p, magics[k]);
-          const uint32_t j = lane & 7u;
-          uint32_t i = class_start(p, R, B, j, ctab);
+    // tier 1 (256 < p <= S/8): batches of 8 primes; lane (slot, class)
+    // computes one class start, then the whole wave strikes each prime with
+    // lane = (class, hit) and stride 8p (loop-invariant word mask).
+    {
+      const uint32_t nb = (t1 - t0 + 7u) >> 3;
+      uint32_t b = wave;
+      uint32_t kN = t0 + b * 8u + slot;
+      uint32_t pN = (b < nb && kN < t1) ? primes[kN] : 0u;
+      uint64_t MN = (b < nb && kN < t1) ? magics[kN] : 0ull;
+      for (; b < nb; b += nwaves) {
+        const uint32_t p = pN;
+        const uint64_t M = MN;
+        const uint32_t bn = b + nwaves;
+        kN = t0 + bn * 8u + slot;
+        pN = (bn < nb && kN < t1) ? primes[kN] : 0u;
+        MN = (bn < nb && kN < t1) ? magics[kN] : 0ull;
+        const uint32_t st = p ? class_start(p, mod_magic(B, p, M), B, j, ctab) : 0u;
+        for (uint32_t q = 0; q < 8; q++) {
+          const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
+          if (!ps) break;
+          const uint32_t pos = (uint32_t) __shfl((int) st, (int) (q * 8u + j)) + slot * ps;
+          const uint32_t mask = ~(1u << (((pos & 3u) << 3) + j));
+          const uint32_t step = 8u * ps;
+          for (uint32_t addr = pos & ~3u; addr < S; addr += step)
+            atomicAnd(&sieve[addr >> 2], mask);
+        }
+      }
+    }
+
+    // tier 2 (S/8 < p <= S): lane = (prime slot, class), stride p; the word
+    // mask rotates by 8 * (p mod 4) bits per step.
+    {
+      const uint32_t nb = (t2 - t1 + 7u) >> 3;
+      uint32_t b = wave;
+      uint32_t kN = t1 + b * 8u + slot;
+      uint32_t pN = (b < nb && kN < t2) ? primes[kN] : 0u;
+      uint64_t MN = (b < nb && kN < t2) ? magics[kN] : 0ull;
+      for (; b < nb; b += nwaves) {
+        const uint32_t p = pN;
+        const uint64_t M = MN;
+        const uint32_t bn = b + nwaves;
+        kN = t1 + bn * 8u + slot;
+        pN = (bn < nb && kN < t2) ? primes[kN] : 0u;
+        MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
+        if (p) {
+          uint32_t i = class_start(p, mod_magic(B, p, M), B, j, ctab);
           uint32_t mask = ~(1u << (((i & 3u) << 3) + j));
           const uint32_t rot = (p & 3u) << 3;
           for (; i < S; i += p) {
@@ -276,15 +325,20 @@ psg_sieve_count(PsgSieveArgs args)
             mask = rotl32(mask, rot);
           }
         }
-      } else {
-        // tier 3: lane = prime, at most one hit per class
-        const uint32_t k = t2 + ((item - n1 - n2) << 6) + lane;
+      }
+    }
+
+    // tier 3 (S < p <= bucket threshold): lane = prime, <= 1 hit per class
+    {
+      const uint32_t nb = (t3 - t2 + 63u) >> 6;
+      for (uint32_t b = wave; b < nb; b += nwaves) {
+        const uint32_t k = t2 + (b << 6) + lane;
         if (k < t3) {
           const uint32_t p = primes[k];
           const uint32_t R = mod_magic(B, p, magics[k]);
-          for (uint32_t j = 0; j < 8; j++) {
-            const uint32_t i = class_start(p, R, B, j, ctab);
-            if (i < S) strike(sieve, i, j);
+          for (uint32_t c = 0; c < 8; c++) {
+            const uint32_t i = class_start(p, R, B, c, ctab);
+            if (i < S) strike(sieve, i, c);
           }
         }
       }
@@ -292,20 +346,32 @@ psg_sieve_count(PsgSieveArgs args)
   }
 
   // ---- 2b. big-prime bucket lists of this segment -----------------------
+  // Four independent loads per thread are in flight before their strikes.
   if (args.buckets) {
     const uint32_t* __restrict__ offs = (const uint32_t*) args.bucketOffs;
     const uint32_t ls = seg - args.slabSegBegin;
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t e0 = min(offs[ls * 8u + j], args.capacity);
-      const uint32_t e1 = min(offs[ls * 8u + j + 1u], args.capacity);
+    for (uint32_t c = 0; c < 8; c++) {
+      const uint32_t e0 = min(offs[ls * 8u + c], args.capacity);
+      const uint32_t e1 = min(offs[ls * 8u + c + 1u], args.capacity);
+      uint32_t e = e0 + tid;
       if (args.entryBytes == 2) {
         const uint16_t* __restrict__ ent = (const uint16_t*) args.buckets;
-        for (uint32_t e = e0 + tid; e < e1; e += nthreads)
-          strike(sieve, ent[e], j);
+        for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
+          const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
+          const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
+          strike(sieve, v0, c); strike(sieve, v1, c);
+          strike(sieve, v2, c); strike(sieve, v3, c);
+        }
+        for (; e < e1; e += nthreads) strike(sieve, ent[e], c);
       } else {
         const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
-        for (uint32_t e = e0 + tid; e < e1; e += nthreads)
-          strike(sieve, ent[e], j);
+        for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
+          const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
+          const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
+          strike(sieve, v0, c); strike(sieve, v1, c);
+          strike(sieve, v2, c); strike(sieve, v3, c);
+        }
+        for (; e < e1; e += nthreads) strike(sieve, ent[e], c);
       }
     }
   }
