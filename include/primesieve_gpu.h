@@ -43,6 +43,22 @@ void psg_set_gpu_work_group_size(int size);
 void psg_set_gpu_segments_per_launch(int segments);
 void psg_set_gpu_presieve(int enable);
 
+/* New: the full tuning vector (the reference's four knobs plus the bucket
+ * tier's).  0 in a field means "derive from the device"; bucket_factor
+ * 0xffffffff disables the bucket tier.  Setting recreates the backend on the
+ * next call, like the reference's setters (src/gpu/GpuSieve.cpp:228-254);
+ * psg_get_gpu_tuning reports the values the active backend uses. */
+typedef struct psg_tuning {
+  uint32_t sieve_bytes;       /* LDS bytes per segment (multiple of 256) */
+  uint32_t work_group_size;   /* threads per workgroup: 64 * waves */
+  uint32_t segments_per_launch;
+  uint32_t slab_segments;     /* segments per big-prime bucket slab */
+  uint32_t bucket_factor;     /* primes > bucket_factor * sieve_bytes use buckets */
+  uint32_t presieve;          /* 1 = pre-sieve tiles for 7..23 */
+} psg_tuning;
+int psg_set_gpu_tuning(const psg_tuning* t);
+int psg_get_gpu_tuning(psg_tuning* t);
+
 /* gpu_active_device / gpu_last_error (reference gpu.hpp:58-63): copy the
  * string into buf (truncated, NUL-terminated); return its full length. */
 int  psg_gpu_active_device(char* buf, size_t len);

@@ -45,7 +45,7 @@ struct GpuConfig
   bool preSieve = true;            // pre-sieve tiles for 7..23
   uint32_t slabSegments = 0;       // segments per big-prime bucket slab
   uint32_t bucketFactor = 0;       // primes > bucketFactor * sieveBytes use the
-                                   // bucket tier; 0 = auto (1), ~0u = never
+                                   // bucket tier; 0 = auto (8), ~0u = never
 };
 
 class GpuBackend

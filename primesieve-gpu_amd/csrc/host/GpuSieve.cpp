@@ -172,6 +172,13 @@ uint64_t countOnGpu(uint64_t start, uint64_t stop, int countType, bool& usedGpu)
   return total;
 }
 
+void setConfig(const GpuConfig& config)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_config = config;
+  g_backend.reset();
+}
+
 std::mutex& gpuMutex() { return g_mutex; }
 std::string& lastErrorRef() { return g_lastError; }
 void resetBackend() { g_backend.reset(); }

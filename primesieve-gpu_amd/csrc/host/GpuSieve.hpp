@@ -15,6 +15,10 @@ namespace primesieve {
 namespace gpu {
 
 class GpuBackend;
+struct GpuConfig;
+
+/// Replace the whole configuration; the backend is recreated on next use.
+void setConfig(const GpuConfig& config);
 
 /// The global lock every public GPU entry point holds.
 std::mutex& gpuMutex();

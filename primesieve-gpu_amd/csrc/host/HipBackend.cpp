@@ -85,14 +85,14 @@ void HipBackend::init()
 
   // geometry: LDS segment, workgroup, launch size
   const uint64_t ldsMax = std::min<uint64_t>(dev_.localMemBytes, PSG_MAX_SEG_BYTES + 1024);
-  uint32_t sb = config_.sieveBytes ? config_.sieveBytes : 64u * 1024u;
+  uint32_t sb = config_.sieveBytes ? config_.sieveBytes : 128u * 1024u;
   sb = (sb / 256u) * 256u;
   if (sb < 256u) sb = 256u;
   if (sb > PSG_MAX_SEG_BYTES) sb = PSG_MAX_SEG_BYTES;
   while (sb > 256u && (uint64_t) sb + 1024u > ldsMax) sb -= 256u;
   sieveBytes_ = sb;
 
-  uint32_t wg = config_.workGroupSize ? config_.workGroupSize : 512u;
+  uint32_t wg = config_.workGroupSize ? config_.workGroupSize : 1024u;
   wg = (wg / 64u) * 64u;
   if (wg < 64u) wg = 64u;
   if (wg > 1024u) wg = 1024u;
@@ -101,7 +101,7 @@ void HipBackend::init()
   wgSize_ = wg;
 
   segmentsPerLaunch_ = config_.segmentsPerLaunch ? config_.segmentsPerLaunch : 65536u;
-  bucketFactor_ = (config_.bucketFactor == 0) ? 1u
+  bucketFactor_ = (config_.bucketFactor == 0) ? 8u
                 : (config_.bucketFactor == 0xffffffffu) ? 0u : config_.bucketFactor;
   // a slab spans ~128 MiB of sieve bytes, at most 2048 segments (the count
   // and fill kernels keep one LDS counter per (segment, class): 64 KiB)

@@ -68,6 +68,19 @@ public:
 
   const psg_stats& lastStats() const { return stats_; }
 
+  /// The geometry actually in use.
+  psg_tuning tuning() const
+  {
+    psg_tuning t;
+    t.sieve_bytes = sieveBytes_;
+    t.work_group_size = wgSize_;
+    t.segments_per_launch = segmentsPerLaunch_;
+    t.slab_segments = slabSegments_;
+    t.bucket_factor = bucketFactor_ ? bucketFactor_ : 0xffffffffu;
+    t.presieve = config_.preSieve ? 1u : 0u;
+    return t;
+  }
+
 private:
   void init();
   void ensureSievingPrimes(uint32_t limit);

@@ -79,6 +79,33 @@ void psg_set_gpu_work_group_size(int size) { primesieve::set_gpu_work_group_size
 void psg_set_gpu_segments_per_launch(int s) { primesieve::set_gpu_segments_per_launch(s); }
 void psg_set_gpu_presieve(int enable) { primesieve::set_gpu_presieve(enable != 0); }
 
+int psg_set_gpu_tuning(const psg_tuning* t)
+{
+  if (!t)
+    return -1;
+  primesieve::gpu::GpuConfig c;
+  c.sieveBytes = t->sieve_bytes;
+  c.workGroupSize = t->work_group_size;
+  c.segmentsPerLaunch = t->segments_per_launch;
+  c.slabSegments = t->slab_segments;
+  c.bucketFactor = t->bucket_factor;
+  c.preSieve = t->presieve != 0;
+  primesieve::gpu::setConfig(c);
+  return 0;
+}
+
+int psg_get_gpu_tuning(psg_tuning* t)
+{
+  if (!t)
+    return -1;
+  std::lock_guard<std::mutex> lock(primesieve::gpu::gpuMutex());
+  psg::HipBackend* b = primesieve::gpu::activeHipBackend();
+  if (!b)
+    return -1;
+  *t = b->tuning();
+  return 0;
+}
+
 int psg_gpu_active_device(char* buf, size_t len)
 {
   return copyOut(primesieve::gpu_active_device(), buf, len);

@@ -80,6 +80,24 @@ __device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
   return (x << r) | (x >> (32u - r));
 }
 
+// Strike one class progression from byte `pos` with word stride `step`
+// (a multiple of 4, so the word mask is loop-invariant) up to byte `hi`
+// (a multiple of 4); four atomics per loop trip.
+__device__ __forceinline__ void strike_run(uint32_t* sieve, uint32_t pos, uint32_t step,
+                                           uint32_t hi, uint32_t bit)
+{
+  const uint32_t mask = ~(1u << (((pos & 3u) << 3) + bit));
+  uint32_t addr = pos & ~3u;
+  for (; addr + 3u * step < hi; addr += 4u * step) {
+    atomicAnd(&sieve[addr >> 2], mask);
+    atomicAnd(&sieve[(addr + step) >> 2], mask);
+    atomicAnd(&sieve[(addr + 2u * step) >> 2], mask);
+    atomicAnd(&sieve[(addr + 3u * step) >> 2], mask);
+  }
+  for (; addr < hi; addr += step)
+    atomicAnd(&sieve[addr >> 2], mask);
+}
+
 // Sum over the 64 lanes of a wave: xor butterflies inside each 32-lane half
 // with ds_swizzle (bitmask mode: and_mask 0x1f, xor_mask 1..16), then the two
 // halves through readlane.
@@ -193,7 +211,7 @@ extern "C" __global__ void __launch_bounds__(1024)
 psg_sieve_count(PsgSieveArgs args)
 {
   extern __shared__ uint32_t sieve[];
-  __shared__ uint16_t ctab[240];
+  __shared__ __attribute__((aligned(16))) uint16_t ctab[240];
   __shared__ uint32_t wsum[PSG_NCOUNTS];
 
   const uint32_t tid = threadIdx.x;
@@ -264,10 +282,7 @@ psg_sieve_count(PsgSieveArgs args)
           const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
           if (!ps) break;
           const uint32_t pos = sliceLo + (uint32_t) __shfl((int) st, (int) (q * 8u + j)) + slot * ps;
-          const uint32_t mask = ~(1u << (((pos & 3u) << 3) + j));
-          const uint32_t step = 8u * ps;
-          for (uint32_t addr = pos & ~3u; addr < sliceHi; addr += step)
-            atomicAnd(&sieve[addr >> 2], mask);
+          strike_run(sieve, pos, 8u * ps, sliceHi, j);
         }
       }
     }
@@ -293,10 +308,7 @@ psg_sieve_count(PsgSieveArgs args)
           const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
           if (!ps) break;
           const uint32_t pos = (uint32_t) __shfl((int) st, (int) (q * 8u + j)) + slot * ps;
-          const uint32_t mask = ~(1u << (((pos & 3u) << 3) + j));
-          const uint32_t step = 8u * ps;
-          for (uint32_t addr = pos & ~3u; addr < S; addr += step)
-            atomicAnd(&sieve[addr >> 2], mask);
+          strike_run(sieve, pos, 8u * ps, S, j);
         }
       }
     }
@@ -318,26 +330,50 @@ psg_sieve_count(PsgSieveArgs args)
         MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
         if (p) {
           uint32_t i = class_start(p, mod_magic(B, p, M), B, j, ctab);
+          // two strikes per trip: the second mask is the first rotated by
+          // 8 * (p mod 4); after two steps the rotation is always 16 bits
           uint32_t mask = ~(1u << (((i & 3u) << 3) + j));
           const uint32_t rot = (p & 3u) << 3;
-          for (; i < S; i += p) {
+          for (; i + p < S; i += 2u * p) {
             atomicAnd(&sieve[i >> 2], mask);
-            mask = rotl32(mask, rot);
+            atomicAnd(&sieve[(i + p) >> 2], rotl32(mask, rot));
+            mask = rotl32(mask, 16u);
           }
+          if (i < S) atomicAnd(&sieve[i >> 2], mask);
         }
       }
     }
 
-    // tier 3 (S < p <= bucket threshold): lane = prime, <= 1 hit per class
+    // tier 3 (S < p <= bucket threshold): lane = prime, <= 1 hit per class.
+    // The class table row of p is one 16-byte LDS read; the prime itself can
+    // only be in range while B < 2^32 (uniform test).
     {
       const uint32_t nb = (t3 - t2 + 63u) >> 6;
-      for (uint32_t b = wave; b < nb; b += nwaves) {
-        const uint32_t k = t2 + (b << 6) + lane;
-        if (k < t3) {
-          const uint32_t p = primes[k];
-          const uint32_t R = mod_magic(B, p, magics[k]);
+      const bool far = B >= 0x100000000ull;
+      uint32_t b = wave;
+      uint32_t kN = t2 + (b << 6) + lane;
+      uint32_t pN = (b < nb && kN < t3) ? primes[kN] : 0u;
+      uint64_t MN = (b < nb && kN < t3) ? magics[kN] : 0ull;
+      for (; b < nb; b += nwaves) {
+        const uint32_t p = pN;
+        const uint64_t M = MN;
+        const uint32_t bn = b + nwaves;
+        kN = t2 + (bn << 6) + lane;
+        pN = (bn < nb && kN < t3) ? primes[kN] : 0u;
+        MN = (bn < nb && kN < t3) ? magics[kN] : 0ull;
+        if (p) {
+          const uint32_t R = mod_magic(B, p, M);
+          const uint32_t a = p / 30u;
+          const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
+          const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
+#pragma unroll
           for (uint32_t c = 0; c < 8; c++) {
-            const uint32_t i = class_start(p, R, B, c, ctab);
+            const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
+            const uint32_t q = ent & 0xffu;
+            const uint32_t cc = a * q + (ent >> 8) - 1u;
+            uint32_t i = (cc >= R) ? cc - R : cc - R + p;
+            if (!far && q == 1u && B + i < (uint64_t) p)
+              i = 0xffffffffu;          // the prime itself; its next hit is beyond S
             if (i < S) strike(sieve, i, c);
           }
         }

@@ -65,6 +65,20 @@ class Roofline(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("sieve_bytes", ctypes.c_uint32),
+        ("work_group_size", ctypes.c_uint32),
+        ("segments_per_launch", ctypes.c_uint32),
+        ("slab_segments", ctypes.c_uint32),
+        ("bucket_factor", ctypes.c_uint32),
+        ("presieve", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -91,6 +105,8 @@ def _declare(lib):
         "psg_gpu_last_stats": (i32, [ctypes.POINTER(Stats)]),
         "psg_algorithmic_work": (i32, [u64, u64, ctypes.c_uint32, ctypes.POINTER(Work)]),
         "psg_measure_roofline": (i32, [ctypes.POINTER(Roofline)]),
+        "psg_set_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
+        "psg_get_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
     }
     for n in COUNT_NAMES:
         sig["psg_gpu_count_" + n] = (u64, [u64, u64])
@@ -152,6 +168,22 @@ def set_segments_per_launch(n):
 
 def set_presieve(enable):
     lib().psg_set_gpu_presieve(1 if enable else 0)
+
+
+def set_tuning(sieve_bytes=0, work_group_size=0, segments_per_launch=0, slab_segments=0,
+               bucket_factor=0, presieve=True):
+    """Set the whole tuning vector (0 = auto); recreates the backend."""
+    t = Tuning(sieve_bytes, work_group_size, segments_per_launch, slab_segments,
+               bucket_factor, 1 if presieve else 0)
+    if lib().psg_set_gpu_tuning(ctypes.byref(t)) != 0:
+        raise PrimesieveError("psg_set_gpu_tuning failed")
+
+
+def get_tuning():
+    t = Tuning()
+    if lib().psg_get_gpu_tuning(ctypes.byref(t)) != 0:
+        raise PrimesieveError(last_error())
+    return t.as_dict()
 
 
 def active_device():
