@@ -89,6 +89,23 @@ int psg_gpu_try_count(uint64_t start, uint64_t stop, int count_type, uint64_t* r
  * -1 on error (see psg_gpu_last_error). */
 int psg_gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6]);
 
+/* ---- batches ------------------------------------------------------------
+ * New (no reference counterpart; the nearest are the random-interval loops of
+ * test/count_primes3.cpp:44-50).  out[i] = count of type k_i over
+ * [start_i, stop_i], k = 1 primes, 2 twins, ..., 6 sextuplets.
+ * PSG_BATCH_PERSISTENT packs all queries into persistent work-stealing
+ * launches; PSG_BATCH_PLAIN runs one count call (launch series + sync) per
+ * query, the baseline the batch is compared with.  0 on success, -1 on error. */
+typedef struct psg_query {
+  uint64_t start;
+  uint64_t stop;
+  uint32_t k;
+  uint32_t reserved;
+} psg_query;
+#define PSG_BATCH_PERSISTENT 0
+#define PSG_BATCH_PLAIN 1
+int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int mode);
+
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart.  Statistics of the last count call, measured
  * with HIP events on the backend's own stream. */

@@ -154,6 +154,49 @@ bool countAllOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6])
   return true;
 }
 
+/// Counts of a batch of queries: out[i] = count of type k_i - 1 in
+/// [start_i, stop_i].  mode 0 packs every query into the persistent engine;
+/// mode 1 is the plain path, one count call (launch series and
+/// synchronisation) per query, kept as the baseline the batch is measured
+/// against.  No reference counterpart.
+bool countBatchOnGpu(const GpuQuery* q, size_t n, uint64_t* out, int mode)
+{
+  for (size_t i = 0; i < n; i++)
+    if (q[i].k < 1 || q[i].k > 6)
+      throw primesieve_error("batch query k must be 1..6");
+  psg::HipBackend* backend = activeHipBackend();
+  if (!backend)
+    return false;
+  std::vector<psg::Interval> iv;
+  iv.reserve(n);
+  for (size_t i = 0; i < n; i++)
+  {
+    out[i] = 0;
+    if (q[i].start > q[i].stop)
+      continue;
+    uint64_t small[6] = { 0, 0, 0, 0, 0, 0 };
+    addSmallPrimes(q[i].start, q[i].stop, small);
+    out[i] = small[q[i].k - 1];
+    if (q[i].stop >= 7)
+      iv.push_back(psg::Interval{ std::max<uint64_t>(q[i].start, 7), q[i].stop, (uint32_t) i });
+  }
+  if (mode == 1)
+  {
+    for (const psg::Interval& v : iv)
+    {
+      uint64_t c[6];
+      backend->countAll(v.lo, v.hi, c);
+      out[v.out] += c[q[v.out].k - 1];
+    }
+    return true;
+  }
+  std::vector<uint64_t> six(n * 6);
+  backend->countBatch(iv, (uint32_t) n, six.data());
+  for (const psg::Interval& v : iv)
+    out[v.out] += six[(size_t) v.out * 6 + q[v.out].k - 1];
+  return true;
+}
+
 /// Reference countOnGpu (src/gpu/GpuSieve.cpp:131-176).
 uint64_t countOnGpu(uint64_t start, uint64_t stop, int countType, bool& usedGpu)
 {
@@ -326,6 +369,14 @@ bool gpu_try_count(uint64_t start, uint64_t stop, int countType, uint64_t& resul
     g_backend.reset();
     return false;
   }
+}
+
+void gpu_count_batch(const std::vector<GpuQuery>& queries, std::vector<uint64_t>& out)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  out.assign(queries.size(), 0);
+  if (!gpu::countBatchOnGpu(queries.data(), queries.size(), out.data(), 0))
+    throw primesieve_error("GPU is not available: " + g_lastError);
 }
 
 void gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6])

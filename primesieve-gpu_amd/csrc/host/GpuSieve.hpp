@@ -9,6 +9,8 @@
 #include <mutex>
 #include <string>
 
+#include <primesieve/gpu.hpp>
+
 namespace psg { class HipBackend; }
 
 namespace primesieve {
@@ -33,6 +35,8 @@ std::string& lastErrorRef();
 void resetBackend();
 /// Six counts of [start, stop] including 2/3/5; false when no GPU is usable.
 bool countAllOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6]);
+/// Batch counts (mode 0 persistent, 1 plain); false when no GPU is usable.
+bool countBatchOnGpu(const GpuQuery* q, size_t n, uint64_t* out, int mode);
 
 } // namespace gpu
 } // namespace primesieve

@@ -66,11 +66,12 @@ HipBackend::~HipBackend()
   tile0_.release();
   tile1_.release();
   counts_.release();
+  descriptors_.release();
+  workCounters_.release();
   bucketCounts_.release();
   listOff_.release();
   entries_.release();
   overflow_.release();
-  if (hostCounts_) (void) api_->HostFree(hostCounts_);
   if (hostOverflow_) (void) api_->HostFree(hostOverflow_);
   for (hipEvent_t e : phaseEvents_) (void) api_->EventDestroy(e);
   if (evStart_) (void) api_->EventDestroy(evStart_);
@@ -107,8 +108,7 @@ void HipBackend::init()
   // and fill kernels keep one LDS counter per (segment, class): 64 KiB)
   uint32_t slab = config_.slabSegments ? config_.slabSegments
                                        : (uint32_t) ((128ull << 20) / sieveBytes_);
-  slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 2048));
-  bucketChunks_ = std::max<uint32_t>(64, 4 * (dev_.computeUnits ? dev_.computeUnits : 256));
+  slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 8192));
   windowSegs_ = 64;
   firstCrossPrime_ = config_.preSieve ? PSG_FIRST_CROSS_PRIME : 7u;
 
@@ -137,11 +137,7 @@ void HipBackend::init()
     hipCheck(api_, api_->Memcpy(tile0_.get(), t0.data(), t0.size() * 4, hipMemcpyHostToDevice), "hipMemcpy tile0");
     hipCheck(api_, api_->Memcpy(tile1_.get(), t1.data(), t1.size() * 4, hipMemcpyHostToDevice), "hipMemcpy tile1");
   }
-  counts_.allocate(api_, PSG_NCOUNTS * sizeof(uint64_t));
   void* hc = nullptr;
-  hipCheck(api_, api_->HostMalloc(&hc, PSG_NCOUNTS * sizeof(uint64_t), 0), "hipHostMalloc");
-  hostCounts_ = (uint64_t*) hc;
-  hc = nullptr;
   hipCheck(api_, api_->HostMalloc(&hc, 4, 0), "hipHostMalloc");
   hostOverflow_ = (uint32_t*) hc;
   *hostOverflow_ = 0;
@@ -173,193 +169,10 @@ void HipBackend::ensureSievingPrimes(uint32_t limit)
   primesLimit_ = (uint32_t) want;
 }
 
-void HipBackend::launchSieve(const PsgSieveArgs& args, uint32_t nseg)
+uint32_t HipBackend::primesUpTo(uint64_t x) const
 {
-  PsgSieveArgs a = args;
-  void* params[] = { &a };
-  hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, nseg, 1, 1, wgSize_, 1, 1,
-                                          sieveBytes_, stream_, params, nullptr),
-           "hipModuleLaunchKernel(psg_sieve_count)");
-}
-
-void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
-{
-  const auto t0 = std::chrono::steady_clock::now();
-  stats_ = psg_stats{};
-  stats_.sieve_bytes = sieveBytes_;
-  stats_.work_group_size = wgSize_;
-  std::fill(out, out + PSG_NCOUNTS, 0);
-  if (lo < 7) lo = 7;
-  if (hi < 7 || lo > hi)
-    return;
-
-  hipCheck(api_, api_->SetDevice(dev_.deviceIndex), "hipSetDevice");
-  const ByteRange br = byteRange(lo, hi);
-  const uint32_t sq = (uint32_t) isqrt64(hi);
-  ensureSievingPrimes(sq);
-
-  const uint32_t S = sieveBytes_;
-  const auto upto = [&](uint64_t x) {
-    return (uint32_t) (std::upper_bound(hostPrimes_.begin(), hostPrimes_.end(),
-                                        (uint32_t) std::min<uint64_t>(x, 0xffffffffull)) -
-                       hostPrimes_.begin());
-  };
-  const uint32_t nprimes = upto(sq);
-
-  PsgSieveArgs args;
-  std::memset(&args, 0, sizeof args);
-  args.range.gLo = br.gLo;
-  args.range.gHi = br.gHi;
-  args.range.loMask = br.loMask;
-  args.range.hiMask = br.hiMask;
-  args.range.nprimes = nprimes;
-  args.primes = primes_.addr();
-  args.magics = magics_.addr();
-  args.tile0 = tile0_.addr();
-  args.tile1 = tile1_.addr();
-  args.counts = counts_.addr();
-  args.segBytes = S;
-  args.tier0End = std::min(upto(std::min<uint32_t>(256, S / 8)), nprimes);
-  args.tier1End = std::min(upto(S / 8), nprimes);
-  args.tier2End = std::min(upto(S), nprimes);
-  args.tier3End = nprimes;
-
-  const uint64_t nseg = (br.gHi - br.gLo) / S + 1;
-
-  // big primes (p > bucket threshold) go through the bucket tier when the
-  // segment size is a power of two; otherwise tier 3 handles every p > S
-  const bool pow2 = (S & (S - 1)) == 0;
-  const uint64_t threshold = (uint64_t) S * bucketFactor_;
-  const uint32_t bigBegin = (pow2 && bucketFactor_) ? std::min(upto(threshold), nprimes) : nprimes;
-  const bool useBuckets = bigBegin < nprimes;
-  args.tier3End = bigBegin;
-
-  hipCheck(api_, api_->MemsetAsync(counts_.get(), 0, PSG_NCOUNTS * sizeof(uint64_t), stream_),
-           "hipMemsetAsync");
-  hipCheck(api_, api_->EventRecord(evStart_, stream_), "hipEventRecord");
-  uint32_t launches = 0;
-  size_t nev = 0;
-  if (!useBuckets)
-  {
-    for (uint64_t s = 0; s < nseg; s += segmentsPerLaunch_)
-    {
-      const uint32_t n = (uint32_t) std::min<uint64_t>(segmentsPerLaunch_, nseg - s);
-      args.range.B0 = br.gLo + s * S;
-      recordPhase(nev++);
-      launchSieve(args, n);
-      launches++;
-    }
-    recordPhase(nev++);
-  }
-  else
-  {
-    uint32_t shift = 0;
-    while ((1u << shift) < S) shift++;
-    const uint32_t L = slabSegments_;
-    const uint32_t nBig = nprimes - bigBegin;
-    const uint32_t entryBytes = (S <= 65536u) ? 2u : 4u;
-    // Each class of each prime hits a slab of B bytes at most B/p + 1 times,
-    // so this capacity is a strict upper bound (the overflow flag is a guard).
-    const double invSum = invPrefix_[nprimes] - invPrefix_[bigBegin];
-    const uint64_t cap64 = (uint64_t) (8.0 * (double) L * S * invSum) + 8ull * nBig + 4096;
-    if (cap64 > 0x7fffffffull)
-      throw primesieve_error("bucket slab too large; lower the slab size");
-    const uint32_t capacity = (uint32_t) cap64;
-    // workgroups = windows x prime chunks, about bucketChunks_ of them
-    const uint32_t R = std::min(windowSegs_, L);
-    const uint32_t nWindowsMax = (L + R - 1) / R;
-    const uint32_t nChunks = std::max<uint32_t>(1, std::min<uint32_t>(
-        (bucketChunks_ + nWindowsMax - 1) / nWindowsMax, (nBig + 63) / 64));
-    const uint32_t nListsMax = 8u * L;
-    bucketCounts_.allocate(api_, (size_t) nChunks * nListsMax * 4);
-    listOff_.allocate(api_, ((size_t) nListsMax + 1) * 4);
-    entries_.allocate(api_, (size_t) capacity * entryBytes);
-    overflow_.allocate(api_, 4);
-    hipCheck(api_, api_->MemsetAsync(overflow_.get(), 0, 4, stream_), "hipMemsetAsync");
-
-    PsgBucketArgs b;
-    std::memset(&b, 0, sizeof b);
-    b.primes = primes_.addr();
-    b.magics = magics_.addr();
-    b.counts = bucketCounts_.addr();
-    b.listOff = listOff_.addr();
-    b.entries = entries_.addr();
-    b.overflow = overflow_.addr();
-    b.segShift = shift;
-    b.bigBegin = bigBegin;
-    b.bigEnd = nprimes;
-    b.nChunks = nChunks;
-    b.capacity = capacity;
-    b.entryBytes = entryBytes;
-    b.windowSegs = R;
-
-    args.buckets = entries_.addr();
-    args.bucketOffs = listOff_.addr();
-    args.entryBytes = entryBytes;
-    args.capacity = capacity;
-    args.segBegin = 0;
-    args.slabSegBegin = 0;
-
-    for (uint64_t s = 0; s < nseg; s += L)
-    {
-      const uint32_t n = (uint32_t) std::min<uint64_t>(L, nseg - s);
-      b.slabB = br.gLo + s * S;
-      b.slabBytes = n * S;
-      b.nLists = 8u * n;
-      b.nWindows = (n + R - 1) / R;
-      const uint32_t grid = b.nWindows * nChunks;
-      const uint32_t ldsBytes = 8u * R * 4u;
-      void* pb[] = { &b };
-      recordPhase(nev++);
-      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketCount_, grid, 1, 1, 512, 1, 1,
-                                              ldsBytes, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_count)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnScanChunks_, (b.nLists + 255) / 256, 1, 1, 256, 1, 1,
-                                              0, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_scan_chunks)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnScanLists_, 1, 1, 1, 1024, 1, 1, 0, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_scan_lists)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, grid, 1, 1, 512, 1, 1,
-                                              ldsBytes, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_fill)");
-      recordPhase(nev++);
-      args.range.B0 = b.slabB;
-      launchSieve(args, n);
-      launches++;
-    }
-    recordPhase(nev++);
-    hipCheck(api_, api_->MemcpyAsync(hostOverflow_, overflow_.get(), 4, hipMemcpyDeviceToHost, stream_),
-             "hipMemcpyAsync overflow");
-    stats_.bucket_entries = 0;
-  }
-  hipCheck(api_, api_->EventRecord(evStop_, stream_), "hipEventRecord");
-  hipCheck(api_, api_->MemcpyAsync(hostCounts_, counts_.get(), PSG_NCOUNTS * sizeof(uint64_t),
-                                   hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync counts");
-  hipCheck(api_, api_->StreamSynchronize(stream_), "sieve kernel");
-  if (useBuckets && *hostOverflow_)
-    throw primesieve_error("big-prime bucket capacity exceeded");
-  float ms = 0.f;
-  hipCheck(api_, api_->EventElapsedTime(&ms, evStart_, evStop_), "hipEventElapsedTime");
-
-  // phase timings: with buckets, events alternate fill | sieve | fill | ...
-  double fillMs = 0, sieveMs = 0;
-  for (size_t e = 0; e + 1 < nev; e++)
-  {
-    float t = 0.f;
-    hipCheck(api_, api_->EventElapsedTime(&t, phaseEvents_[e], phaseEvents_[e + 1]), "hipEventElapsedTime");
-    if (useBuckets && e % 2 == 0) fillMs += t;
-    else sieveMs += t;
-  }
-
-  std::copy(hostCounts_, hostCounts_ + PSG_NCOUNTS, out);
-  stats_.kernel_ms = sieveMs;
-  stats_.fill_ms = fillMs;
-  stats_.segments = nseg;
-  stats_.launches = launches;
-  stats_.sieving_primes = nprimes;
-  stats_.device_ms = ms;
-  stats_.wall_ms = std::chrono::duration<double, std::milli>(
-                       std::chrono::steady_clock::now() - t0).count();
+  const uint32_t v = (uint32_t) std::min<uint64_t>(x, 0xffffffffull);
+  return (uint32_t) (std::upper_bound(hostPrimes_.begin(), hostPrimes_.end(), v) - hostPrimes_.begin());
 }
 
 void HipBackend::recordPhase(size_t i)
@@ -371,6 +184,286 @@ void HipBackend::recordPhase(size_t i)
     phaseEvents_.push_back(e);
   }
   hipCheck(api_, api_->EventRecord(phaseEvents_[i], stream_), "hipEventRecord");
+}
+
+uint32_t HipBackend::persistentGrid(uint32_t nSegs)
+{
+  if (!persistentWGs_)
+  {
+    int perCU = 0;
+    if (api_->ModuleOccupancy(&perCU, fnSieve_, (int) wgSize_, sieveBytes_) != hipSuccess || perCU < 1)
+      perCU = 1;
+    persistentWGs_ = (uint32_t) perCU * (dev_.computeUnits ? dev_.computeUnits : 256);
+  }
+  return std::max<uint32_t>(1, std::min(nSegs, persistentWGs_));
+}
+
+/// Sieve every interval; out6[slot * 6 + t] receives the six counts of the
+/// intervals whose `out` is slot.  Intervals are cut into runs of at most
+/// opt.runSegs segments, runs are packed into slabs, and each slab is one
+/// bucket build (when big primes exist) plus one persistent sieve launch.
+void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6,
+                         const ExecOptions& opt)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  stats_ = psg_stats{};
+  stats_.sieve_bytes = sieveBytes_;
+  stats_.work_group_size = wgSize_;
+  std::fill(out6, out6 + (size_t) outs * PSG_NCOUNTS, 0);
+  hipCheck(api_, api_->SetDevice(dev_.deviceIndex), "hipSetDevice");
+
+  uint64_t maxHi = 0;
+  for (const Interval& v : iv)
+    if (v.lo <= v.hi && v.hi >= 7) maxHi = std::max(maxHi, v.hi);
+  if (!maxHi)
+    return;
+  ensureSievingPrimes((uint32_t) isqrt64(maxHi));
+
+  const uint32_t S = sieveBytes_;
+  uint32_t shift = 0;
+  while ((1u << shift) < S) shift++;
+  const bool pow2 = (1u << shift) == S;
+  const uint32_t bigBegin = (pow2 && bucketFactor_)
+                          ? primesUpTo((uint64_t) S * bucketFactor_) : (uint32_t) hostPrimes_.size();
+  const uint32_t entryBytes = (S <= 65536u) ? 2u : 4u;
+
+  // ---- runs -----------------------------------------------------------------
+  std::vector<PsgRun> runs;
+  uint64_t segsTotal = 0;
+  for (const Interval& v : iv)
+  {
+    const uint64_t lo = std::max<uint64_t>(v.lo, 7);
+    if (v.hi < 7 || lo > v.hi)
+      continue;
+    const ByteRange br = byteRange(lo, v.hi);
+    const uint32_t np = primesUpTo(isqrt64(v.hi));
+    const uint64_t nseg = (br.gHi - br.gLo) / S + 1;
+    for (uint64_t s = 0; s < nseg; s += opt.runSegs)
+    {
+      PsgRun r;
+      std::memset(&r, 0, sizeof r);
+      r.B0 = br.gLo + s * S;
+      r.gLo = br.gLo;
+      r.gHi = br.gHi;
+      r.loMask = br.loMask;
+      r.hiMask = br.hiMask;
+      r.nprimes = np;
+      r.out = v.out;
+      r.nseg = (uint32_t) std::min<uint64_t>(opt.runSegs, nseg - s);
+      runs.push_back(r);
+    }
+    segsTotal += nseg;
+  }
+  if (runs.empty())
+    return;
+
+  // ---- slabs and windows ------------------------------------------------------
+  struct Slab { size_t run0, run1, win0, win1; uint32_t nSegs, nWGs, countsWords, capacity, maxLists; };
+  std::vector<Slab> slabs;
+  std::vector<PsgWindow> wins;
+  const double hBig = invPrefix_[bigBegin];
+  const auto expectedHits = [&](uint32_t bytes, uint32_t bigEnd) {
+    return 8.0 * bytes * (invPrefix_[bigEnd] - hBig);
+  };
+  {
+    Slab cur{ 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    double est = 0;
+    for (size_t i = 0; i < runs.size(); i++)
+    {
+      PsgRun& r = runs[i];
+      const bool big = r.nprimes > bigBegin;
+      const double e = big ? expectedHits(r.nseg * S, r.nprimes) : 0.0;
+      if (cur.run1 > cur.run0 &&
+          (cur.nSegs + r.nseg > opt.slabSegs || est + e > opt.entryBudget))
+      {
+        cur.win1 = wins.size();
+        cur.capacity = (uint32_t) std::min(1.25 * est + 65536.0 + 64.0 * (cur.win1 - cur.win0), 2.0e9);
+        slabs.push_back(cur);
+        cur = Slab{ i, i, wins.size(), wins.size(), 0, 0, 0, 0, 0 };
+        est = 0;
+      }
+      r.segStart = cur.nSegs;
+      if (big)
+      {
+        const uint32_t nBig = r.nprimes - bigBegin;
+        const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
+            opt.maxChunks, (nBig + opt.primesPerChunk - 1) / opt.primesPerChunk));
+        for (uint32_t s = 0; s < r.nseg; s += opt.windowSegs)
+        {
+          PsgWindow w;
+          w.segs = std::min(opt.windowSegs, r.nseg - s);
+          w.base = r.B0 + (uint64_t) s * S;
+          w.firstList = 8u * (cur.nSegs + s);
+          w.bigEnd = r.nprimes;
+          w.nChunks = chunks;
+          w.countsBase = cur.countsWords;
+          w.wgBase = cur.nWGs;
+          cur.countsWords += chunks * 8u * w.segs;
+          cur.nWGs += chunks;
+          cur.maxLists = std::max(cur.maxLists, 8u * w.segs);
+          wins.push_back(w);
+        }
+      }
+      est += e;
+      cur.nSegs += r.nseg;
+      cur.run1 = i + 1;
+    }
+    cur.win1 = wins.size();
+    cur.capacity = (uint32_t) std::min(1.25 * est + 65536.0 + 64.0 * (cur.win1 - cur.win0), 2.0e9);
+    slabs.push_back(cur);
+  }
+
+  // ---- device buffers and one upload of every descriptor ------------------------
+  uint32_t maxSegs = 0, maxCounts = 0, maxCap = 0;
+  for (const Slab& sl : slabs)
+  {
+    maxSegs = std::max(maxSegs, sl.nSegs);
+    maxCounts = std::max(maxCounts, sl.countsWords);
+    maxCap = std::max(maxCap, sl.capacity);
+  }
+  const size_t runBytes = runs.size() * sizeof(PsgRun);
+  const size_t winBytes = wins.size() * sizeof(PsgWindow);
+  descriptors_.allocate(api_, runBytes + winBytes + 16);
+  staging_.resize(runBytes + winBytes);
+  std::memcpy(staging_.data(), runs.data(), runBytes);
+  if (winBytes) std::memcpy(staging_.data() + runBytes, wins.data(), winBytes);
+  hipCheck(api_, api_->MemcpyAsync(descriptors_.get(), staging_.data(), runBytes + winBytes,
+                                   hipMemcpyHostToDevice, stream_), "hipMemcpyAsync descriptors");
+  counts_.allocate(api_, (size_t) outs * PSG_NCOUNTS * sizeof(uint64_t));
+  workCounters_.allocate(api_, slabs.size() * 4);
+  listOff_.allocate(api_, ((size_t) 8 * maxSegs + 1) * 4);
+  if (maxCounts)
+  {
+    bucketCounts_.allocate(api_, (size_t) maxCounts * 4);
+    entries_.allocate(api_, (size_t) maxCap * entryBytes);
+  }
+  overflow_.allocate(api_, 4);
+  hipCheck(api_, api_->MemsetAsync(counts_.get(), 0, (size_t) outs * PSG_NCOUNTS * 8, stream_), "hipMemsetAsync");
+  hipCheck(api_, api_->MemsetAsync(workCounters_.get(), 0, slabs.size() * 4, stream_), "hipMemsetAsync");
+  hipCheck(api_, api_->MemsetAsync(overflow_.get(), 0, 4, stream_), "hipMemsetAsync");
+
+  PsgSieveArgs args;
+  std::memset(&args, 0, sizeof args);
+  args.primes = primes_.addr();
+  args.magics = magics_.addr();
+  args.tile0 = tile0_.addr();
+  args.tile1 = tile1_.addr();
+  args.counts = counts_.addr();
+  args.listOff = listOff_.addr();
+  args.segBytes = S;
+  args.tier0End = primesUpTo(std::min<uint32_t>(PSG_TIER0_MAX_PRIME, S / 8));
+  args.tier1End = primesUpTo(S / 8);
+  args.tier2End = primesUpTo(S);
+  args.tier3End = bigBegin;
+  args.entryBytes = entryBytes;
+
+  PsgBucketArgs b;
+  std::memset(&b, 0, sizeof b);
+  b.primes = primes_.addr();
+  b.magics = magics_.addr();
+  b.counts = bucketCounts_.addr();
+  b.listOff = listOff_.addr();
+  b.entries = entries_.addr();
+  b.overflow = overflow_.addr();
+  b.segShift = shift;
+  b.bigBegin = bigBegin;
+  b.entryBytes = entryBytes;
+
+  hipCheck(api_, api_->EventRecord(evStart_, stream_), "hipEventRecord");
+  size_t nev = 0;
+  std::vector<bool> fillPhase;
+  for (size_t si = 0; si < slabs.size(); si++)
+  {
+    const Slab& sl = slabs[si];
+    if (sl.win1 > sl.win0)
+    {
+      b.windows = descriptors_.addr() + runBytes + sl.win0 * sizeof(PsgWindow);
+      b.nWindows = (uint32_t) (sl.win1 - sl.win0);
+      b.nWGs = sl.nWGs;
+      b.nLists = 8u * sl.nSegs;
+      b.capacity = sl.capacity;
+      b.maxWindowLists = sl.maxLists;
+      void* pb[] = { &b };
+      recordPhase(nev++);
+      fillPhase.push_back(true);
+      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketCount_, sl.nWGs, 1, 1, 512, 1, 1,
+                                              sl.maxLists * 4, stream_, pb, nullptr),
+               "hipModuleLaunchKernel(psg_bucket_count)");
+      hipCheck(api_, api_->ModuleLaunchKernel(fnScanChunks_, (b.nLists + 255) / 256, 1, 1, 256, 1, 1,
+                                              0, stream_, pb, nullptr),
+               "hipModuleLaunchKernel(psg_bucket_scan_chunks)");
+      hipCheck(api_, api_->ModuleLaunchKernel(fnScanLists_, 1, 1, 1, 1024, 1, 1, 0, stream_, pb, nullptr),
+               "hipModuleLaunchKernel(psg_bucket_scan_lists)");
+      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, sl.nWGs, 1, 1, 512, 1, 1,
+                                              sl.maxLists * 4, stream_, pb, nullptr),
+               "hipModuleLaunchKernel(psg_bucket_fill)");
+      args.buckets = entries_.addr();
+      args.capacity = sl.capacity;
+    }
+    else
+    {
+      args.buckets = 0;
+      args.capacity = 0;
+    }
+    args.runs = descriptors_.addr() + sl.run0 * sizeof(PsgRun);
+    args.nRuns = (uint32_t) (sl.run1 - sl.run0);
+    args.nSegs = sl.nSegs;
+    args.workCounter = workCounters_.addr() + si * 4;
+    recordPhase(nev++);
+    fillPhase.push_back(false);
+    void* pa[] = { &args };
+    hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, persistentGrid(sl.nSegs), 1, 1, wgSize_, 1, 1,
+                                            sieveBytes_, stream_, pa, nullptr),
+             "hipModuleLaunchKernel(psg_sieve_count)");
+  }
+  recordPhase(nev++);
+  hipCheck(api_, api_->EventRecord(evStop_, stream_), "hipEventRecord");
+  hostCounts_.resize((size_t) outs * PSG_NCOUNTS);
+  hipCheck(api_, api_->MemcpyAsync(hostOverflow_, overflow_.get(), 4, hipMemcpyDeviceToHost, stream_),
+           "hipMemcpyAsync overflow");
+  hipCheck(api_, api_->MemcpyAsync(hostCounts_.data(), counts_.get(), hostCounts_.size() * 8,
+                                   hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync counts");
+  hipCheck(api_, api_->StreamSynchronize(stream_), "sieve kernels");
+  if (*hostOverflow_)
+    throw primesieve_error("big-prime bucket capacity exceeded");
+
+  float ms = 0.f;
+  hipCheck(api_, api_->EventElapsedTime(&ms, evStart_, evStop_), "hipEventElapsedTime");
+  double fillMs = 0, sieveMs = 0;
+  for (size_t e = 0; e + 1 < nev; e++)
+  {
+    float t = 0.f;
+    hipCheck(api_, api_->EventElapsedTime(&t, phaseEvents_[e], phaseEvents_[e + 1]), "hipEventElapsedTime");
+    if (fillPhase[e]) fillMs += t;
+    else sieveMs += t;
+  }
+  std::copy(hostCounts_.begin(), hostCounts_.end(), out6);
+  stats_.kernel_ms = sieveMs;
+  stats_.fill_ms = fillMs;
+  stats_.device_ms = ms;
+  stats_.segments = segsTotal;
+  stats_.launches = (uint32_t) slabs.size();
+  stats_.sieving_primes = primesUpTo(isqrt64(maxHi));
+  stats_.wall_ms = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t0).count();
+}
+
+void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
+{
+  ExecOptions opt;
+  opt.runSegs = slabSegments_;
+  opt.slabSegs = slabSegments_;
+  opt.windowSegs = windowSegs_;
+  execute({ Interval{ lo, hi, 0 } }, 1, out, opt);
+}
+
+void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6)
+{
+  ExecOptions opt;
+  opt.runSegs = std::min<uint32_t>(2048, std::max<uint32_t>(slabSegments_, 1));
+  opt.windowSegs = opt.runSegs;          // one prime sweep per run
+  opt.slabSegs = std::max<uint32_t>(opt.runSegs, batchSlabSegs_);
+  execute(iv, outs, out6, opt);
 }
 
 uint64_t HipBackend::count(uint64_t lo, uint64_t hi, int countType)

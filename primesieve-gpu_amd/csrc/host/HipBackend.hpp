@@ -2,9 +2,14 @@
 /// @file  HipBackend.hpp
 /// @brief gfx950 implementation of GpuBackend.  Replaces the reference's
 ///        CudaBackend (src/gpu/cuda/CudaBackend.cpp:222-491): device set-up,
-///        code-object load, sieving-prime cache, the launch loop and the host
-///        reduction -- here the six counts are reduced on the device and one
-///        48-byte copy ends a call.
+///        code-object load, sieving-prime cache, the launch loop and the
+///        reduction -- here the counts are reduced on the device and one small
+///        copy ends a call.
+///
+///        One execution engine serves both the reference's single-interval
+///        count and the batch API: intervals are cut into runs of segments,
+///        runs are packed into slabs, and each slab is a big-prime bucket
+///        build plus one persistent, work-stealing sieve launch.
 ///
 #ifndef PSG_HIP_BACKEND_HPP
 #define PSG_HIP_BACKEND_HPP
@@ -43,12 +48,23 @@ private:
   size_t bytes_ = 0;
 };
 
-/// One query of a batch after host-side planning.
-struct BatchItem
+/// An interval [lo, hi] whose six counts are added to output slot `out`.
+struct Interval
 {
-  uint64_t start;
-  uint64_t stop;
-  uint32_t k;        // 1..6
+  uint64_t lo;
+  uint64_t hi;
+  uint32_t out;
+};
+
+/// How the engine cuts work (all in segments).
+struct ExecOptions
+{
+  uint32_t runSegs = 1024;        // longest run
+  uint32_t slabSegs = 1024;       // segments per slab (one persistent launch)
+  uint32_t windowSegs = 64;       // segments per bucket window
+  uint32_t primesPerChunk = 2048; // big primes per count / fill workgroup ...
+  uint32_t maxChunks = 64;        // ... but at most this many workgroups per window
+  double entryBudget = 5.0e8;     // expected bucket entries per slab
 };
 
 class HipBackend : public primesieve::gpu::GpuBackend
@@ -65,6 +81,10 @@ public:
 
   /// All six counts of [lo, hi] (lo >= 7 after clamping) in one pass.
   void countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS]);
+
+  /// Six counts per output slot for a batch of intervals (lo >= 7 after
+  /// clamping), all in one launch series.
+  void countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6);
 
   const psg_stats& lastStats() const { return stats_; }
 
@@ -84,7 +104,10 @@ public:
 private:
   void init();
   void ensureSievingPrimes(uint32_t limit);
-  void launchSieve(const PsgSieveArgs& args, uint32_t nseg);
+  void execute(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6,
+               const ExecOptions& opt);
+  uint32_t primesUpTo(uint64_t x) const;
+  uint32_t persistentGrid(uint32_t nSegs);
   void recordPhase(size_t i);
 
   const HipApi* api_;
@@ -93,10 +116,11 @@ private:
   uint32_t sieveBytes_ = 0;
   uint32_t wgSize_ = 0;
   uint32_t segmentsPerLaunch_ = 0;
-  uint32_t bucketFactor_ = 1;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
-  uint32_t slabSegments_ = 0;     // segments per bucket slab
-  uint32_t bucketChunks_ = 0;     // target workgroups of the bucket count / fill kernels
-  uint32_t windowSegs_ = 64;      // segments per bucket window
+  uint32_t bucketFactor_ = 8;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
+  uint32_t slabSegments_ = 0;     // segments per slab of a single interval
+  uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
+  uint32_t windowSegs_ = 64;      // segments per bucket window of a single interval
+  uint32_t persistentWGs_ = 0;    // resident workgroups of the sieve kernel
 
   hipModule_t module_ = nullptr;
   hipFunction_t fnSieve_ = nullptr;
@@ -108,17 +132,20 @@ private:
   hipEvent_t evStart_ = nullptr;
   hipEvent_t evStop_ = nullptr;
 
-  DeviceBuffer primes_;      // uint32 sieving primes >= first cross prime
-  DeviceBuffer magics_;      // uint64 division magics
+  DeviceBuffer primes_;        // uint32 sieving primes >= first cross prime
+  DeviceBuffer magics_;        // uint64 division magics
   DeviceBuffer tile0_;
   DeviceBuffer tile1_;
-  DeviceBuffer counts_;      // uint64[6]
-  uint64_t* hostCounts_ = nullptr;   // pinned
-  DeviceBuffer bucketCounts_;        // uint32 [chunks][lists]
-  DeviceBuffer listOff_;             // uint32 [lists + 1]
-  DeviceBuffer entries_;             // bucket entries
-  DeviceBuffer overflow_;            // uint32 flag
-  uint32_t* hostOverflow_ = nullptr; // pinned
+  DeviceBuffer counts_;        // uint64 [outputs][6]
+  DeviceBuffer descriptors_;   // runs and windows of every slab of a call
+  DeviceBuffer workCounters_;  // uint32 per slab
+  DeviceBuffer bucketCounts_;  // uint32 counts blocks
+  DeviceBuffer listOff_;       // uint32 [lists + 1]
+  DeviceBuffer entries_;       // bucket entries
+  DeviceBuffer overflow_;      // uint32 flag
+  uint32_t* hostOverflow_ = nullptr;   // pinned
+  std::vector<uint64_t> hostCounts_;
+  std::vector<unsigned char> staging_;
   std::vector<hipEvent_t> phaseEvents_;
 
   std::vector<uint32_t> hostPrimes_; // the primes held in primes_

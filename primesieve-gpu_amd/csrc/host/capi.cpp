@@ -13,6 +13,7 @@
 
 #include <primesieve/gpu.hpp>
 #include <primesieve_gpu.h>
+#include <primesieve/primesieve_error.hpp>
 
 #include <cerrno>
 #include <cstring>
@@ -155,6 +156,28 @@ int psg_gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6])
   catch (const std::exception& e)
   {
     setLastError(e.what());
+    return -1;
+  }
+}
+
+int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int mode)
+{
+  if ((!queries || !out) && n)
+    return -1;
+  try
+  {
+    std::vector<primesieve::GpuQuery> q(n);
+    for (size_t i = 0; i < n; i++)
+      q[i] = primesieve::GpuQuery{ queries[i].start, queries[i].stop, queries[i].k };
+    std::lock_guard<std::mutex> lock(primesieve::gpu::gpuMutex());
+    if (!primesieve::gpu::countBatchOnGpu(q.data(), n, out, mode))
+      throw primesieve::primesieve_error("GPU is not available: " + primesieve::gpu::lastErrorRef());
+    return 0;
+  }
+  catch (const std::exception& e)
+  {
+    std::lock_guard<std::mutex> lock(primesieve::gpu::gpuMutex());
+    primesieve::gpu::lastErrorRef() = e.what();
     return -1;
   }
 }

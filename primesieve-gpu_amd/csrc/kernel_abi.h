@@ -7,6 +7,12 @@
  * doc/GPU.md:84-88): bit b of global byte g stands for the integer
  * 30*g + {7,11,13,17,19,23,29,31}[b].  A segment is `segBytes` consecutive
  * global bytes held in LDS as 32-bit little-endian words.
+ *
+ * Work is described by runs: a run is a stretch of consecutive segments of
+ * one interval (a whole query, or a piece of a long interval).  A slab is a
+ * list of runs whose segments are numbered 0 .. nSegs-1 in run order; one
+ * slab is sieved by one persistent launch of psg_sieve_count, after its
+ * big-prime bucket lists have been built.
  */
 #ifndef PSG_KERNEL_ABI_H
 #define PSG_KERNEL_ABI_H
@@ -18,70 +24,83 @@
 #define PSG_TILE0_PERIOD 17017u  /* 7*11*13*17 bytes */
 #define PSG_TILE1_PERIOD 437u    /* 19*23 bytes */
 #define PSG_FIRST_CROSS_PRIME 29u
+#define PSG_TIER0_MAX_PRIME 256u
 
-/* One contiguous byte range [gLo, gHi] counted by one launch series. */
-typedef struct PsgRange {
-  uint64_t B0;          /* global byte index of segment 0 */
-  uint64_t gLo;         /* first counted global byte */
-  uint64_t gHi;         /* last counted global byte */
-  uint32_t loMask;      /* bits of byte gLo that are >= lo */
-  uint32_t hiMask;      /* bits of byte gHi that are <= hi */
-  uint32_t nprimes;     /* sieving primes (>= 29) with p*p <= hi */
-  uint32_t pad0;
-} PsgRange;
+/* One run (48 bytes). */
+typedef struct PsgRun {
+  uint64_t B0;          /* global byte of the run's first segment */
+  uint64_t gLo;         /* first counted global byte of the interval */
+  uint64_t gHi;         /* last counted global byte of the interval */
+  uint32_t loMask;      /* bits of byte gLo whose values are >= lo */
+  uint32_t hiMask;      /* bits of byte gHi whose values are <= hi */
+  uint32_t nprimes;     /* sieving primes (from primes[0]) with p*p <= hi */
+  uint32_t out;         /* output slot: counts go to out * PSG_NCOUNTS + type */
+  uint32_t segStart;    /* first slab segment of this run */
+  uint32_t nseg;        /* segments in this run */
+} PsgRun;
 
-/* Arguments of psg_sieve_count: sieve segments B0 + s*segBytes for
- * s in [segBegin, segBegin + gridDim.x) and add the six counts. */
+/* Arguments of psg_sieve_count (persistent: workgroups take slab segments
+ * from *workCounter until nSegs is reached). */
 typedef struct PsgSieveArgs {
-  PsgRange range;
-  uint64_t primes;      /* const uint32_t*: sieving primes >= 29, ascending */
+  uint64_t runs;        /* const PsgRun*, sorted by segStart */
+  uint64_t primes;      /* const uint32_t*: sieving primes, ascending */
   uint64_t magics;      /* const uint64_t*: ceil(2^(64+L)/p), L = floor(log2 p) */
-  uint64_t tile0;       /* const uint32_t*: sliding windows, PSG_TILE0_PERIOD entries */
-  uint64_t tile1;       /* const uint32_t*: sliding windows, PSG_TILE1_PERIOD entries */
-  uint64_t counts;      /* uint64_t[PSG_NCOUNTS], accumulated atomically */
-  uint64_t buckets;     /* const uint32_t*: big-prime bucket entries (0 = none) */
-  uint64_t bucketOffs;  /* const uint32_t*: per-segment entry offsets within the slab */
+  uint64_t tile0;       /* const uint32_t*: sliding windows (0 = no pre-sieve) */
+  uint64_t tile1;       /* const uint32_t*: sliding windows */
+  uint64_t counts;      /* uint64_t[outs][PSG_NCOUNTS], accumulated atomically */
+  uint64_t buckets;     /* bucket entries of the slab (0 = none) */
+  uint64_t listOff;     /* const uint32_t*[8 * nSegs + 1] */
+  uint64_t workCounter; /* uint32_t*, zero at launch */
+  uint32_t nRuns;
+  uint32_t nSegs;       /* segments in the slab */
   uint32_t segBytes;    /* multiple of 256, <= PSG_MAX_SEG_BYTES */
-  uint32_t segBegin;    /* first segment index of this launch */
+  uint32_t tier0End;    /* primes[0, tier0End): p <= 256, struck per wave slice */
   uint32_t tier1End;    /* primes[tier0End, tier1End): p <= segBytes/8 */
   uint32_t tier2End;    /* primes[tier1End, tier2End): p <= segBytes */
-  uint32_t tier3End;    /* primes[tier2End, tier3End): per-segment big primes */
-  uint32_t slabSegBegin;/* first segment index covered by the bucket slab */
+  uint32_t tier3End;    /* primes[tier2End, tier3End): per-segment big primes;
+                           larger primes come from the bucket lists */
   uint32_t entryBytes;  /* 2: uint16 byte offsets (segBytes <= 65536); 4: uint32 */
   uint32_t capacity;    /* bucket entries allocated */
-  uint32_t tier0End;    /* primes[0, tier0End): p <= 256, struck per wave slice */
+  uint32_t pad;
 } PsgSieveArgs;
 
-/* Big-prime bucket tier.  A slab is `slabBytes / segBytes` consecutive
- * segments starting at global byte slabB.  Every prime in
- * primes[bigBegin, bigEnd) is walked over the whole slab once per phase;
- * its hits go to list (segment * 8 + class), class = bit index.  Lists are
- * laid out by a counting sort with no global atomics:
- *   psg_bucket_count  per-workgroup LDS histogram -> counts[chunk][list]
- *   psg_bucket_scan_chunks  exclusive prefix over chunks per list, totals
- *   psg_bucket_scan_lists   exclusive prefix over lists -> listOff
- *   psg_bucket_fill   re-walk, LDS cursors, store byte offsets
- * The sieve kernel then strikes list (s*8 + j) of segment s with bit j. */
+/* One bucket window (32 bytes): consecutive segments of one run walked by
+ * nChunks workgroups, each over an interleaved share of the big primes
+ * primes[bigBegin, bigEnd). */
+typedef struct PsgWindow {
+  uint64_t base;        /* global byte of the window's first byte */
+  uint32_t firstList;   /* slab list index of its list 0 (8 * slab segment) */
+  uint32_t segs;        /* segments in the window */
+  uint32_t bigEnd;      /* exclusive prime index bound of its interval */
+  uint32_t nChunks;     /* workgroups walking this window */
+  uint32_t countsBase;  /* offset of its counts block [nChunks][8 * segs] */
+  uint32_t wgBase;      /* first workgroup of the window in the count / fill grid */
+} PsgWindow;
+
+/* Big-prime bucket tier of one slab.  Hits go to list (segment * 8 + class),
+ * class = bit index.  Lists are laid out by a counting sort with no global
+ * atomics:
+ *   psg_bucket_count        per-workgroup LDS histogram -> counts
+ *   psg_bucket_scan_chunks  exclusive prefix over a window's chunks per list
+ *   psg_bucket_scan_lists   exclusive prefix over all lists -> listOff
+ *   psg_bucket_fill         re-walk with LDS cursors, store byte offsets
+ * psg_sieve_count then strikes list (s*8 + j) of segment s with bit j. */
 typedef struct PsgBucketArgs {
-  uint64_t slabB;       /* global byte index of the slab's first byte */
+  uint64_t windows;     /* const PsgWindow*, sorted by wgBase (and firstList) */
   uint64_t primes;      /* const uint32_t* */
   uint64_t magics;      /* const uint64_t* */
-  uint64_t counts;      /* uint32_t[nChunks][nLists] */
+  uint64_t counts;      /* uint32_t counts blocks */
   uint64_t listOff;     /* uint32_t[nLists + 1] */
   uint64_t entries;     /* uint16_t* or uint32_t* (entryBytes) */
   uint64_t overflow;    /* uint32_t flag, set when capacity is exceeded */
-  uint32_t slabBytes;   /* bytes covered, multiple of segBytes, < 2^31 */
-  uint32_t segShift;    /* log2(segBytes) */
+  uint32_t nWindows;
+  uint32_t nWGs;        /* workgroups of the count / fill grid */
   uint32_t nLists;      /* 8 * segments in the slab */
+  uint32_t segShift;    /* log2(segBytes) */
   uint32_t bigBegin;
-  uint32_t bigEnd;
-  uint32_t nChunks;     /* prime chunks (primes interleaved over chunks) */
   uint32_t capacity;    /* entries allocated */
   uint32_t entryBytes;  /* 2 or 4 */
-  uint32_t windowSegs;  /* segments per window: workgroup (w, c) walks the
-                           primes of chunk c over window w only, so that its
-                           sub-lists are contiguous and its LDS counters few */
-  uint32_t nWindows;    /* ceil(segments in slab / windowSegs) */
+  uint32_t maxWindowLists; /* LDS counters per workgroup */
 } PsgBucketArgs;
 
 #endif

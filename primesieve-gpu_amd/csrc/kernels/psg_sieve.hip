@@ -114,7 +114,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 
 // Zero the bytes of an 8-byte sieve word outside [gLo, gHi] and apply the
 // partial-byte masks of the range ends.  gw = global byte of the word's byte 0.
-__device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRange& r)
+__device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRun& r)
 {
   if (gw > r.gHi || gw + 7 < r.gLo)
     return 0;
@@ -152,83 +152,23 @@ __device__ void init_ctab(uint16_t* ctab)
   }
 }
 
-// Workgroup (w, c) of the count / fill kernels: window w of the slab
-// (windowSegs segments), prime chunk c.  Primes are interleaved over the
-// chunks (k = bigBegin + c + i * nChunks) so that small and large big primes
-// are spread evenly.
-struct BucketWindow
+// One segment: fill, strike, count.  Ends with a barrier, so that the LDS
+// sieve and the wave sums may be reused by the next segment.
+__device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const PsgRun& run,
+                                              uint32_t seg, uint32_t* sieve,
+                                              const uint16_t* ctab, uint32_t* wsum)
 {
-  uint32_t w, c;
-  uint32_t firstList;   // global list index of the window's list 0
-  uint32_t nLists;      // lists of this window (8 * its segments)
-  uint32_t bytes;       // bytes of this window
-  uint64_t base;        // global byte of the window's first byte
-};
-
-__device__ __forceinline__ BucketWindow bucket_window(const PsgBucketArgs& a)
-{
-  BucketWindow bw;
-  bw.w = blockIdx.x % a.nWindows;
-  bw.c = blockIdx.x / a.nWindows;
-  const uint32_t seg0 = bw.w * a.windowSegs;
-  const uint32_t segs = min(a.windowSegs, (a.slabBytes >> a.segShift) - seg0);
-  bw.firstList = seg0 * 8u;
-  bw.nLists = segs * 8u;
-  bw.bytes = segs << a.segShift;
-  bw.base = a.slabB + ((uint64_t) seg0 << a.segShift);
-  return bw;
-}
-
-// Walk every hit of the chunk's primes over the window; call
-// emit(windowList, byteOffsetInSegment) per hit.
-template <typename Emit>
-__device__ __forceinline__ void bucket_walk(const PsgBucketArgs& a, const BucketWindow& bw,
-                                            const uint16_t* ctab, Emit emit)
-{
-  const uint32_t* __restrict__ primes = (const uint32_t*) a.primes;
-  const uint64_t* __restrict__ magics = (const uint64_t*) a.magics;
-  const uint32_t segMask = (1u << a.segShift) - 1u;
-  const uint32_t stride = blockDim.x * a.nChunks;
-  const uint32_t bytes = bw.bytes;
-  for (uint32_t k = a.bigBegin + bw.c + threadIdx.x * a.nChunks; k < a.bigEnd; k += stride) {
-    const uint32_t p = primes[k];
-    const uint32_t R = mod_magic(bw.base, p, magics[k]);
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t i = class_start(p, R, bw.base, j, ctab);
-      if (p >= 0x80000000u) {
-        if (i < bytes) emit(((i >> a.segShift) << 3) + j, i & segMask);
-      } else {
-        for (uint32_t pos = i; pos < bytes; pos += p)
-          emit(((pos >> a.segShift) << 3) + j, pos & segMask);
-      }
-    }
-  }
-}
-
-}  // namespace
-
-extern "C" __global__ void __launch_bounds__(1024)
-psg_sieve_count(PsgSieveArgs args)
-{
-  extern __shared__ uint32_t sieve[];
-  __shared__ __attribute__((aligned(16))) uint16_t ctab[240];
-  __shared__ uint32_t wsum[PSG_NCOUNTS];
-
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
   const uint32_t lane = tid & 63u;
   const uint32_t S = args.segBytes;
   const uint32_t words = S >> 2;
-  const uint32_t seg = args.segBegin + blockIdx.x;
-  const uint64_t B = args.range.B0 + (uint64_t) seg * S;
+  const uint64_t B = run.B0 + (uint64_t) (seg - run.segStart) * S;
 
   const uint32_t* __restrict__ primes = (const uint32_t*) args.primes;
   const uint64_t* __restrict__ magics = (const uint64_t*) args.magics;
   const uint32_t* __restrict__ tile0 = (const uint32_t*) args.tile0;
   const uint32_t* __restrict__ tile1 = (const uint32_t*) args.tile1;
-
-  init_ctab(ctab);
-  if (tid < PSG_NCOUNTS) wsum[tid] = 0;
 
   // ---- 1. fill from the pre-sieve tiles ----------------------------------
   if (!tile0) {
@@ -257,7 +197,7 @@ psg_sieve_count(PsgSieveArgs args)
   // and the next batch's primes are loaded before the current batch is
   // struck, so that global-load latency overlaps LDS work.
   {
-    const uint32_t np = args.range.nprimes;
+    const uint32_t np = run.nprimes;
     const uint32_t t0 = min(args.tier0End, np);
     const uint32_t t1 = max(t0, min(args.tier1End, np));
     const uint32_t t2 = max(t1, min(args.tier2End, np));
@@ -384,11 +324,10 @@ psg_sieve_count(PsgSieveArgs args)
   // ---- 2b. big-prime bucket lists of this segment -----------------------
   // Four independent loads per thread are in flight before their strikes.
   if (args.buckets) {
-    const uint32_t* __restrict__ offs = (const uint32_t*) args.bucketOffs;
-    const uint32_t ls = seg - args.slabSegBegin;
+    const uint32_t* __restrict__ offs = (const uint32_t*) args.listOff;
     for (uint32_t c = 0; c < 8; c++) {
-      const uint32_t e0 = min(offs[ls * 8u + c], args.capacity);
-      const uint32_t e1 = min(offs[ls * 8u + c + 1u], args.capacity);
+      const uint32_t e0 = min(offs[seg * 8u + c], args.capacity);
+      const uint32_t e1 = min(offs[seg * 8u + c + 1u], args.capacity);
       uint32_t e = e0 + tid;
       if (args.entryBytes == 2) {
         const uint16_t* __restrict__ ent = (const uint16_t*) args.buckets;
@@ -421,8 +360,8 @@ psg_sieve_count(PsgSieveArgs args)
     for (uint32_t w = tid; w < words64; w += nthreads) {
       uint64_t x = s64[w];
       const uint64_t gw = B + 8ull * w;
-      if (gw <= args.range.gLo || gw + 7 >= args.range.gHi)
-        x = mask_edges(x, gw, args.range);
+      if (gw <= run.gLo || gw + 7 >= run.gHi)
+        x = mask_edges(x, gw, run);
       const uint64_t t1 = x & (x >> 1);
       const uint64_t t2 = t1 & (x >> 2);
       const uint64_t t3 = t2 & (x >> 3);
@@ -444,12 +383,90 @@ psg_sieve_count(PsgSieveArgs args)
   }
   __syncthreads();
   if (tid < PSG_NCOUNTS && wsum[tid])
-    atomicAdd((unsigned long long*) args.counts + tid, (unsigned long long) wsum[tid]);
+    atomicAdd((unsigned long long*) args.counts + (uint64_t) run.out * PSG_NCOUNTS + tid,
+              (unsigned long long) wsum[tid]);
+}
+
+// Last index i in [0, n) with key(i) <= x (keys ascending, key(0) <= x).
+template <typename Key>
+__device__ __forceinline__ uint32_t last_le(uint32_t n, uint32_t x, Key key)
+{
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (key(mid) <= x) lo = mid; else hi = mid - 1u;
+  }
+  return lo;
+}
+
+// Walk every hit of window w's chunk-c primes over the window; call
+// emit(windowList, byteOffsetInSegment) per hit.  Primes are interleaved over
+// the chunks (k = bigBegin + c + i * nChunks) so that small and large big
+// primes are spread evenly.
+template <typename Emit>
+__device__ __forceinline__ void bucket_walk(const PsgBucketArgs& a, const PsgWindow& w, uint32_t c,
+                                            const uint16_t* ctab, Emit emit)
+{
+  const uint32_t* __restrict__ primes = (const uint32_t*) a.primes;
+  const uint64_t* __restrict__ magics = (const uint64_t*) a.magics;
+  const uint32_t segMask = (1u << a.segShift) - 1u;
+  const uint32_t stride = blockDim.x * w.nChunks;
+  const uint32_t bytes = w.segs << a.segShift;
+  for (uint32_t k = a.bigBegin + c + threadIdx.x * w.nChunks; k < w.bigEnd; k += stride) {
+    const uint32_t p = primes[k];
+    const uint32_t R = mod_magic(w.base, p, magics[k]);
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t i = class_start(p, R, w.base, j, ctab);
+      if (p >= 0x80000000u) {
+        if (i < bytes) emit(((i >> a.segShift) << 3) + j, i & segMask);
+      } else {
+        for (uint32_t pos = i; pos < bytes; pos += p)
+          emit(((pos >> a.segShift) << 3) + j, pos & segMask);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ const PsgWindow& window_of_wg(const PsgBucketArgs& a, uint32_t& chunk)
+{
+  const PsgWindow* win = (const PsgWindow*) a.windows;
+  const uint32_t w = last_le(a.nWindows, blockIdx.x, [&](uint32_t i) { return win[i].wgBase; });
+  chunk = blockIdx.x - win[w].wgBase;
+  return win[w];
+}
+
+}  // namespace
+
+// Persistent segment kernel: each workgroup takes slab segments from
+// *workCounter (work stealing) until all nSegs are done.
+extern "C" __global__ void __launch_bounds__(1024)
+psg_sieve_count(PsgSieveArgs args)
+{
+  extern __shared__ uint32_t sieve[];
+  __shared__ __attribute__((aligned(16))) uint16_t ctab[240];
+  __shared__ uint32_t wsum[PSG_NCOUNTS];
+  __shared__ uint32_t sSeg, sRun;
+
+  const PsgRun* runs = (const PsgRun*) args.runs;
+  init_ctab(ctab);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const uint32_t seg = atomicAdd((uint32_t*) args.workCounter, 1u);
+      sSeg = seg;
+      sRun = (seg < args.nSegs)
+           ? last_le(args.nRuns, seg, [&](uint32_t i) { return runs[i].segStart; }) : 0u;
+    }
+    if (threadIdx.x < PSG_NCOUNTS) wsum[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t seg = sSeg;
+    if (seg >= args.nSegs) break;
+    const PsgRun run = runs[sRun];
+    sieve_segment(args, run, seg, sieve, ctab, wsum);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Big-prime bucket tier (see PsgBucketArgs): counting sort of the slab's hits
-// into per-(segment, class) lists.
+// Big-prime bucket tier (see PsgBucketArgs)
 // ---------------------------------------------------------------------------
 
 extern "C" __global__ void __launch_bounds__(512)
@@ -458,28 +475,37 @@ psg_bucket_count(PsgBucketArgs a)
   extern __shared__ uint32_t hist[];
   __shared__ uint16_t ctab[240];
   init_ctab(ctab);
-  const BucketWindow bw = bucket_window(a);
-  for (uint32_t l = threadIdx.x; l < bw.nLists; l += blockDim.x) hist[l] = 0;
+  uint32_t c;
+  const PsgWindow w = window_of_wg(a, c);
+  const uint32_t nl = w.segs * 8u;
+  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) hist[l] = 0;
   __syncthreads();
-  bucket_walk(a, bw, ctab, [&](uint32_t list, uint32_t) { atomicAdd(&hist[list], 1u); });
+  bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t) { atomicAdd(&hist[list], 1u); });
   __syncthreads();
-  uint32_t* counts = (uint32_t*) a.counts + (uint64_t) bw.c * a.nLists + bw.firstList;
-  for (uint32_t l = threadIdx.x; l < bw.nLists; l += blockDim.x) counts[l] = hist[l];
+  uint32_t* counts = (uint32_t*) a.counts + w.countsBase + (uint64_t) c * nl;
+  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) counts[l] = hist[l];
 }
 
-// counts[c][l] <- sum_{c' < c} counts[c'][l];  listOff[l] <- sum_c counts[c][l]
+// Per slab list l: counts[c][l] <- sum_{c' < c} counts[c'][l] within its
+// window, listOff[l] <- total (0 for lists of runs without big primes).
 extern "C" __global__ void __launch_bounds__(256)
 psg_bucket_scan_chunks(PsgBucketArgs a)
 {
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= a.nLists) return;
-  uint32_t* counts = (uint32_t*) a.counts;
+  const PsgWindow* win = (const PsgWindow*) a.windows;
   uint32_t run = 0;
-  for (uint32_t c = 0; c < a.nChunks; c++) {
-    const uint64_t idx = (uint64_t) c * a.nLists + l;
-    const uint32_t v = counts[idx];
-    counts[idx] = run;
-    run += v;
+  if (a.nWindows && win[0].firstList <= l) {
+    const PsgWindow& w = win[last_le(a.nWindows, l, [&](uint32_t i) { return win[i].firstList; })];
+    const uint32_t nl = w.segs * 8u;
+    if (l < w.firstList + nl) {
+      uint32_t* counts = (uint32_t*) a.counts + w.countsBase + (l - w.firstList);
+      for (uint32_t c = 0; c < w.nChunks; c++) {
+        const uint32_t v = counts[(uint64_t) c * nl];
+        counts[(uint64_t) c * nl] = run;
+        run += v;
+      }
+    }
   }
   ((uint32_t*) a.listOff)[l] = run;
 }
@@ -492,7 +518,7 @@ psg_bucket_scan_lists(PsgBucketArgs a)
   uint32_t* off = (uint32_t*) a.listOff;
   const uint32_t n = a.nLists;
   const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-  const uint32_t b = threadIdx.x * per;
+  const uint32_t b = min(threadIdx.x * per, n);
   const uint32_t e = min(b + per, n);
   uint32_t sum = 0;
   for (uint32_t i = b; i < e; i++) sum += off[i];
@@ -523,21 +549,23 @@ psg_bucket_fill(PsgBucketArgs a)
   extern __shared__ uint32_t cursor[];
   __shared__ uint16_t ctab[240];
   init_ctab(ctab);
-  const BucketWindow bw = bucket_window(a);
-  const uint32_t* counts = (const uint32_t*) a.counts + (uint64_t) bw.c * a.nLists + bw.firstList;
-  const uint32_t* listOff = (const uint32_t*) a.listOff + bw.firstList;
-  for (uint32_t l = threadIdx.x; l < bw.nLists; l += blockDim.x) cursor[l] = listOff[l] + counts[l];
+  uint32_t c;
+  const PsgWindow w = window_of_wg(a, c);
+  const uint32_t nl = w.segs * 8u;
+  const uint32_t* counts = (const uint32_t*) a.counts + w.countsBase + (uint64_t) c * nl;
+  const uint32_t* listOff = (const uint32_t*) a.listOff + w.firstList;
+  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) cursor[l] = listOff[l] + counts[l];
   __syncthreads();
   const uint32_t cap = a.capacity;
   if (a.entryBytes == 2) {
     uint16_t* ent = (uint16_t*) a.entries;
-    bucket_walk(a, bw, ctab, [&](uint32_t list, uint32_t off) {
+    bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t off) {
       const uint32_t slot = atomicAdd(&cursor[list], 1u);
       if (slot < cap) ent[slot] = (uint16_t) off;
     });
   } else {
     uint32_t* ent = (uint32_t*) a.entries;
-    bucket_walk(a, bw, ctab, [&](uint32_t list, uint32_t off) {
+    bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t off) {
       const uint32_t slot = atomicAdd(&cursor[list], 1u);
       if (slot < cap) ent[slot] = off;
     });

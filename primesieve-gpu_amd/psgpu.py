@@ -65,6 +65,15 @@ class Roofline(ctypes.Structure):
     ]
 
 
+class Query(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_uint64), ("stop", ctypes.c_uint64),
+                ("k", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+BATCH_PERSISTENT = 0
+BATCH_PLAIN = 1
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [
         ("sieve_bytes", ctypes.c_uint32),
@@ -106,6 +115,7 @@ def _declare(lib):
         "psg_algorithmic_work": (i32, [u64, u64, ctypes.c_uint32, ctypes.POINTER(Work)]),
         "psg_measure_roofline": (i32, [ctypes.POINTER(Roofline)]),
         "psg_set_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
+        "psg_gpu_count_batch": (i32, [ctypes.POINTER(Query), sz, ctypes.POINTER(u64), i32]),
         "psg_get_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
     }
     for n in COUNT_NAMES:
@@ -212,6 +222,19 @@ def count_all(start, stop):
     out = (ctypes.c_uint64 * 6)()
     if lib().psg_gpu_count_all(start, stop, out) != 0:
         raise PrimesieveError(last_error() or "GPU count failed")
+    return list(out)
+
+
+def count_batch(queries, mode=BATCH_PERSISTENT):
+    """queries: sequence of (start, stop, k) with k = 1..6, or an (n, 3)
+    uint64 array.  Returns a list of counts."""
+    n = len(queries)
+    arr = (Query * n)()
+    for i, (a, b, k) in enumerate(queries):
+        arr[i].start, arr[i].stop, arr[i].k = int(a), int(b), int(k)
+    out = (ctypes.c_uint64 * n)()
+    if lib().psg_gpu_count_batch(arr, n, out, mode) != 0:
+        raise PrimesieveError(last_error() or "batch count failed")
     return list(out)
 
 

@@ -1,0 +1,68 @@
+"""GPU parity of the batch API (persistent work-stealing launches and the
+plain one-call-per-query path) against the reference's counts and the
+oracle.  Batch queries are (start, stop, k) with k = 1 primes .. 6
+sextuplets."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _queries():
+    q = np.load(os.path.join(GOLDEN, "config3_queries.npy"), allow_pickle=False)
+    exp = json.load(open(os.path.join(GOLDEN, "config3_expected.json")))
+    idx = sorted(int(i) for i in exp)
+    return [(int(q[i, 0]), int(q[i, 1]), int(q[i, 2])) for i in idx], [exp[str(i)] for i in idx]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_config3_subset_matches_reference(gpu, mode):
+    """384 fixture queries (start <= 1e14) with reference-computed counts."""
+    qs, want = _queries()
+    print("queries", len(qs), "mode", mode, flush=True)
+    got = gpu.count_batch(qs, mode=mode)
+    bad = [(q, g, w) for q, g, w in zip(qs, got, want) if g != w]
+    assert not bad, bad[:5]
+
+
+def test_small_and_edge_queries_against_oracle(gpu, oracle):
+    rng = random.Random(23)
+    qs = [(10, 9, 1), (0, 0, 1), (0, 1, 2), (2, 2, 1), (3, 5, 2), (5, 17, 5), (0, 30, 6),
+          (7, 7, 1), (31, 31, 1)]
+    for base in [0, 1000, 10 ** 6, 10 ** 9, 2 ** 32, 10 ** 12]:
+        for _ in range(12):
+            a = base + rng.randrange(10 ** 5)
+            qs.append((a, a + rng.choice([0, 1, 29, 30, rng.randrange(10 ** 4), rng.randrange(10 ** 7)]),
+                       rng.randrange(1, 7)))
+    got = gpu.count_batch(qs)
+    for (a, b, k), g in zip(qs, got):
+        assert g == oracle.count6(a, b)[k - 1], (a, b, k, g)
+    assert gpu.count_batch(qs, mode=1) == got
+
+
+def test_batch_equals_plain_on_fixture_prefix(gpu):
+    """Size-independent check on the full-size query shape: the persistent
+    engine and the plain per-query path agree on the first 256 config-3
+    queries (starts up to 1e18, lengths up to 2^32)."""
+    q = np.load(os.path.join(GOLDEN, "config3_queries.npy"), allow_pickle=False)[:256]
+    qs = [(int(a), int(b), int(k)) for a, b, k in q]
+    got = gpu.count_batch(qs)
+    plain = gpu.count_batch(qs, mode=1)
+    assert got == plain
+
+
+def test_batch_rejects_bad_k(gpu):
+    with pytest.raises(gpu.PrimesieveError):
+        gpu.count_batch([(0, 100, 0)])
+    with pytest.raises(gpu.PrimesieveError):
+        gpu.count_batch([(0, 100, 7)])
+
+
+def test_empty_batch(gpu):
+    assert gpu.count_batch([]) == []
