@@ -286,12 +286,14 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
       if (big)
       {
         const uint32_t nBig = r.nprimes - bigBegin;
-        const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
-            opt.maxChunks, (nBig + opt.primesPerChunk - 1) / opt.primesPerChunk));
         for (uint32_t s = 0; s < r.nseg; s += opt.windowSegs)
         {
           PsgWindow w;
           w.segs = std::min(opt.windowSegs, r.nseg - s);
+          // enough chunks to spread the primes, but a bounded counts block
+          const uint32_t cap = std::max(opt.maxChunks, opt.chunkListBudget / (8u * w.segs));
+          const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
+              cap, (nBig + opt.primesPerChunk - 1) / opt.primesPerChunk));
           w.base = r.B0 + (uint64_t) s * S;
           w.firstList = 8u * (cur.nSegs + s);
           w.bigEnd = r.nprimes;
@@ -389,7 +391,7 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
       hipCheck(api_, api_->ModuleLaunchKernel(fnBucketCount_, sl.nWGs, 1, 1, 512, 1, 1,
                                               sl.maxLists * 4, stream_, pb, nullptr),
                "hipModuleLaunchKernel(psg_bucket_count)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnScanChunks_, (b.nLists + 255) / 256, 1, 1, 256, 1, 1,
+      hipCheck(api_, api_->ModuleLaunchKernel(fnScanChunks_, (b.nLists + 3) / 4, 1, 1, 256, 1, 1,
                                               0, stream_, pb, nullptr),
                "hipModuleLaunchKernel(psg_bucket_scan_chunks)");
       hipCheck(api_, api_->ModuleLaunchKernel(fnScanLists_, 1, 1, 1, 1024, 1, 1, 0, stream_, pb, nullptr),
@@ -462,6 +464,8 @@ void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint
   ExecOptions opt;
   opt.runSegs = std::min<uint32_t>(2048, std::max<uint32_t>(slabSegments_, 1));
   opt.windowSegs = opt.runSegs;          // one prime sweep per run
+  opt.primesPerChunk = 16384;
+  opt.chunkListBudget = 1u << 19;        // <= 2 MiB of counts per window
   opt.slabSegs = std::max<uint32_t>(opt.runSegs, batchSlabSegs_);
   execute(iv, outs, out6, opt);
 }

@@ -63,7 +63,8 @@ struct ExecOptions
   uint32_t slabSegs = 1024;       // segments per slab (one persistent launch)
   uint32_t windowSegs = 64;       // segments per bucket window
   uint32_t primesPerChunk = 2048; // big primes per count / fill workgroup ...
-  uint32_t maxChunks = 64;        // ... but at most this many workgroups per window
+  uint32_t maxChunks = 64;        // ... but at most max(maxChunks,
+  uint32_t chunkListBudget = 0;   //     chunkListBudget / lists) workgroups per window
   double entryBudget = 5.0e8;     // expected bucket entries per slab
 };
 

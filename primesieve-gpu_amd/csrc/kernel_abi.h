@@ -73,7 +73,7 @@ typedef struct PsgWindow {
   uint32_t segs;        /* segments in the window */
   uint32_t bigEnd;      /* exclusive prime index bound of its interval */
   uint32_t nChunks;     /* workgroups walking this window */
-  uint32_t countsBase;  /* offset of its counts block [nChunks][8 * segs] */
+  uint32_t countsBase;  /* offset of its counts block [8 * segs][nChunks] (list-major) */
   uint32_t wgBase;      /* first workgroup of the window in the count / fill grid */
 } PsgWindow;
 

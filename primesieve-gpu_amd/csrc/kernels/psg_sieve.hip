@@ -399,10 +399,28 @@ __device__ __forceinline__ uint32_t last_le(uint32_t n, uint32_t x, Key key)
   return lo;
 }
 
+// Residues r = (v - 7) mod 30 of the 8 bit positions {0,4,6,10,12,16,22,24},
+// mapped to their bit index: 3 bits per residue, r = 0..19 and r = 20..29.
+constexpr uint64_t kBitLo = (1ull << 12) | (2ull << 18) | (3ull << 30) | (4ull << 36) | (5ull << 48);
+constexpr uint64_t kBitHi = (6ull << 6) | (7ull << 12);
+// bit q set iff gcd(q, 30) = 1, for q = 0..29
+constexpr uint32_t kCoprime30 = 0x208A2882u;
+
+__device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
+{
+  return (uint32_t) ((r < 20u ? (kBitLo >> (3u * r)) : (kBitHi >> (3u * (r - 20u)))) & 7u);
+}
+
 // Walk every hit of window w's chunk-c primes over the window; call
-// emit(windowList, byteOffsetInSegment) per hit.  Primes are interleaved over
-// the chunks (k = bigBegin + c + i * nChunks) so that small and large big
-// primes are spread evenly.
+// emit(windowList, byteOffsetInSegment) per hit.  The big primes are cut into
+// tiles of blockDim consecutive primes and the tiles are dealt round-robin to
+// the chunks (tile m*nChunks + c), so that every wave loads consecutive primes
+// (coalesced) and every chunk gets a similar mix of small and large primes.
+//   dense primes (>= 8 multiples in the window): the 8 residue classes, each
+//     an arithmetic progression of bytes with stride p;
+//   sparse primes: the multiples of p in the window's value range directly,
+//     keeping those whose cofactor is coprime to 30 -- one small division
+//     instead of eight class starts.
 template <typename Emit>
 __device__ __forceinline__ void bucket_walk(const PsgBucketArgs& a, const PsgWindow& w, uint32_t c,
                                             const uint16_t* ctab, Emit emit)
@@ -412,14 +430,32 @@ __device__ __forceinline__ void bucket_walk(const PsgBucketArgs& a, const PsgWin
   const uint32_t segMask = (1u << a.segShift) - 1u;
   const uint32_t stride = blockDim.x * w.nChunks;
   const uint32_t bytes = w.segs << a.segShift;
-  for (uint32_t k = a.bigBegin + c + threadIdx.x * w.nChunks; k < w.bigEnd; k += stride) {
+  const uint64_t V = 30ull * bytes;               // values covered by the window
+  const uint64_t v0 = 30ull * w.base + 7u;        // its first value
+  for (uint32_t k = a.bigBegin + c * blockDim.x + threadIdx.x; k < w.bigEnd; k += stride) {
     const uint32_t p = primes[k];
     const uint32_t R = mod_magic(w.base, p, magics[k]);
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t i = class_start(p, R, w.base, j, ctab);
-      if (p >= 0x80000000u) {
-        if (i < bytes) emit(((i >> a.segShift) << 3) + j, i & segMask);
-      } else {
+    if ((uint64_t) p * 8u > V) {
+      // v0 = 30*Q*p + x with x = 30R + 7 < 31p; the first multiple of p at or
+      // after v0 is v0 + off with cofactor 30*Q + ceil(x / p)
+      const uint64_t x = 30ull * R + 7u;
+      uint32_t t = (uint32_t) ((float) x / (float) p);
+      while ((uint64_t) t * p > x) t--;
+      while ((uint64_t) (t + 1u) * p <= x) t++;
+      const uint32_t u = (uint32_t) (x - (uint64_t) t * p);
+      uint64_t off = u ? (uint64_t) (p - u) : 0ull;
+      uint32_t qm = (t + (u ? 1u : 0u)) % 30u;
+      for (; off < V; off += p) {
+        if (((kCoprime30 >> qm) & 1u) && v0 + off != (uint64_t) p) {
+          const uint32_t byte = (uint32_t) (off / 30u);
+          const uint32_t bit = bit_of_residue((uint32_t) (off - 30ull * byte));
+          emit(((byte >> a.segShift) << 3) + bit, byte & segMask);
+        }
+        qm = (qm == 29u) ? 0u : qm + 1u;
+      }
+    } else {
+      for (uint32_t j = 0; j < 8; j++) {
+        const uint32_t i = class_start(p, R, w.base, j, ctab);
         for (uint32_t pos = i; pos < bytes; pos += p)
           emit(((pos >> a.segShift) << 3) + j, pos & segMask);
       }
@@ -482,32 +518,41 @@ psg_bucket_count(PsgBucketArgs a)
   __syncthreads();
   bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t) { atomicAdd(&hist[list], 1u); });
   __syncthreads();
-  uint32_t* counts = (uint32_t*) a.counts + w.countsBase + (uint64_t) c * nl;
-  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) counts[l] = hist[l];
+  // list-major counts block: counts[l * nChunks + c]
+  uint32_t* counts = (uint32_t*) a.counts + w.countsBase + c;
+  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) counts[(uint64_t) l * w.nChunks] = hist[l];
 }
 
-// Per slab list l: counts[c][l] <- sum_{c' < c} counts[c'][l] within its
-// window, listOff[l] <- total (0 for lists of runs without big primes).
+// One wave per slab list l: counts[l][c] <- sum_{c' < c} counts[l][c'] over
+// its window's chunks (a contiguous row), listOff[l] <- the row total (0 for
+// lists of runs without big primes).  Rows are scanned 64 chunks at a time
+// with a shuffle prefix sum.
 extern "C" __global__ void __launch_bounds__(256)
 psg_bucket_scan_chunks(PsgBucketArgs a)
 {
-  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t l = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (l >= a.nLists) return;
   const PsgWindow* win = (const PsgWindow*) a.windows;
-  uint32_t run = 0;
+  uint32_t carry = 0;
   if (a.nWindows && win[0].firstList <= l) {
     const PsgWindow& w = win[last_le(a.nWindows, l, [&](uint32_t i) { return win[i].firstList; })];
-    const uint32_t nl = w.segs * 8u;
-    if (l < w.firstList + nl) {
-      uint32_t* counts = (uint32_t*) a.counts + w.countsBase + (l - w.firstList);
-      for (uint32_t c = 0; c < w.nChunks; c++) {
-        const uint32_t v = counts[(uint64_t) c * nl];
-        counts[(uint64_t) c * nl] = run;
-        run += v;
+    if (l < w.firstList + w.segs * 8u) {
+      uint32_t* row = (uint32_t*) a.counts + w.countsBase + (uint64_t) (l - w.firstList) * w.nChunks;
+      for (uint32_t c0 = 0; c0 < w.nChunks; c0 += 64u) {
+        const uint32_t c = c0 + lane;
+        const uint32_t v = (c < w.nChunks) ? row[c] : 0u;
+        uint32_t x = v;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+          const uint32_t y = (uint32_t) __shfl_up((int) x, d);
+          if (lane >= d) x += y;
+        }
+        if (c < w.nChunks) row[c] = carry + x - v;
+        carry += (uint32_t) __builtin_amdgcn_readlane((int) x, 63);
       }
     }
   }
-  ((uint32_t*) a.listOff)[l] = run;
+  if (lane == 0) ((uint32_t*) a.listOff)[l] = carry;
 }
 
 // listOff[0..nLists] <- exclusive prefix sum of the list totals (one block)
@@ -552,9 +597,10 @@ psg_bucket_fill(PsgBucketArgs a)
   uint32_t c;
   const PsgWindow w = window_of_wg(a, c);
   const uint32_t nl = w.segs * 8u;
-  const uint32_t* counts = (const uint32_t*) a.counts + w.countsBase + (uint64_t) c * nl;
+  const uint32_t* counts = (const uint32_t*) a.counts + w.countsBase + c;
   const uint32_t* listOff = (const uint32_t*) a.listOff + w.firstList;
-  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) cursor[l] = listOff[l] + counts[l];
+  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x)
+    cursor[l] = listOff[l] + counts[(uint64_t) l * w.nChunks];
   __syncthreads();
   const uint32_t cap = a.capacity;
   if (a.entryBytes == 2) {
