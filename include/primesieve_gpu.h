@@ -152,6 +152,32 @@ typedef struct psg_roofline {
 } psg_roofline;
 int psg_measure_roofline(psg_roofline* out);
 
+/* ---- cost model ---------------------------------------------------------
+ * No reference counterpart (the reference tunes by grid search only,
+ * tools/gpu_check.cpp:164-204).  An FP32 6 -> 64 -> 64 -> 1 MLP (ReLU, ReLU,
+ * linear) with hand-written HIP forward, backward and Adam kernels; inputs
+ * are 6 normalised features per sample (row-major, n x 6), outputs one float
+ * per sample.  Parameters are one flat array of psg_mlp_param_count() floats:
+ * W1[64][6], b1[64], W2[64][64], b2[64], W3[64], b3.  Loss: mean squared
+ * error.  Adam: beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected.
+ * int-returning calls: 0 on success, -1 on error (psg_gpu_last_error). */
+typedef struct psg_mlp psg_mlp;
+int      psg_mlp_param_count(void);
+psg_mlp* psg_mlp_create(void);        /* parameters zero; NULL on error */
+void     psg_mlp_destroy(psg_mlp* m);
+int      psg_mlp_set_params(psg_mlp* m, const float* params);   /* resets Adam state */
+int      psg_mlp_get_params(psg_mlp* m, float* params);
+int      psg_mlp_forward(psg_mlp* m, const float* x, size_t n, float* y);
+/* gradients of the mean squared error over the n samples, and that loss */
+int      psg_mlp_gradients(psg_mlp* m, const float* x, const float* t, size_t n,
+                           float* grads, float* loss);
+/* one Adam update of the parameters with the given gradients */
+int      psg_mlp_adam_step(psg_mlp* m, const float* grads, float lr);
+/* minibatch training: `epochs` shuffled passes (seeded), one Adam step per
+ * batch; epoch_loss[e] (optional) = mean batch loss of epoch e */
+int      psg_mlp_train(psg_mlp* m, const float* x, const float* t, size_t n, uint32_t epochs,
+                       uint32_t batch, float lr, uint32_t seed, float* epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

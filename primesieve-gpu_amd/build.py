@@ -31,9 +31,10 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-para
              "-D__HIP_PLATFORM_AMD__=1", "-I" + os.path.join(ROOT, "include"),
              "-I" + ROCM_INC]
 
-KERNELS = {"psg_sieve": "PSG_SIEVE_HSACO", "psg_bench": "PSG_BENCH_HSACO"}
+KERNELS = {"psg_sieve": "PSG_SIEVE_HSACO", "psg_bench": "PSG_BENCH_HSACO",
+           "psg_mlp": "PSG_MLP_HSACO"}
 HOST_SOURCES = ["hip_api.cpp", "tables.cpp", "HipBackend.cpp", "GpuSieve.cpp", "capi.cpp",
-                "roofline.cpp"]
+                "roofline.cpp", "costmodel.cpp"]
 
 
 def _run_all(cmds):

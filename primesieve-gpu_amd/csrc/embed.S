@@ -17,4 +17,12 @@ psg_bench_hsaco_start:
     .globl psg_bench_hsaco_end
 psg_bench_hsaco_end:
     .byte 0
+    .p2align 12
+    .globl psg_mlp_hsaco_start
+    .type psg_mlp_hsaco_start, @object
+psg_mlp_hsaco_start:
+    .incbin PSG_MLP_HSACO
+    .globl psg_mlp_hsaco_end
+psg_mlp_hsaco_end:
+    .byte 0
     .section .note.GNU-stack,"",@progbits

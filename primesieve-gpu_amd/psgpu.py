@@ -116,6 +116,20 @@ def _declare(lib):
         "psg_measure_roofline": (i32, [ctypes.POINTER(Roofline)]),
         "psg_set_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
         "psg_gpu_count_batch": (i32, [ctypes.POINTER(Query), sz, ctypes.POINTER(u64), i32]),
+        "psg_mlp_param_count": (i32, []),
+        "psg_mlp_create": (ctypes.c_void_p, []),
+        "psg_mlp_destroy": (None, [ctypes.c_void_p]),
+        "psg_mlp_set_params": (i32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+        "psg_mlp_get_params": (i32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+        "psg_mlp_forward": (i32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), sz,
+                                  ctypes.POINTER(ctypes.c_float)]),
+        "psg_mlp_gradients": (i32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_float), sz,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+        "psg_mlp_adam_step": (i32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_float]),
+        "psg_mlp_train": (i32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float), sz, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]),
         "psg_get_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
     }
     for n in COUNT_NAMES:

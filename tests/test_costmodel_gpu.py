@@ -1,0 +1,100 @@
+"""GPU checks of the cost model: the HIP forward, backward and Adam kernels
+against the float64 oracle (oracle/mlp_oracle.py), training that lowers the
+loss, and -- on the committed timing dataset measured on an MI355X -- the
+regret of the trained tuner against grid search and its gain over the fixed
+default geometry."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp_oracle as mo   # noqa: E402
+
+DATA = os.path.join(ROOT, "primesieve-gpu_amd", "data", "costmodel_timings.npz")
+
+# FP32 kernels vs float64 oracle: relative tolerance on outputs / gradients
+RTOL = 2e-4
+ATOL = 2e-5
+
+
+@pytest.fixture(scope="module")
+def cm(gpu):
+    import costmodel
+    return costmodel
+
+
+def _setup(seed, n=300):
+    rng = np.random.default_rng(seed)
+    p = (mo.init_params(seed) + rng.normal(0, 0.02, mo.NPARAMS)).astype(np.float32)
+    X = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    T = rng.normal(0, 1, n).astype(np.float32)
+    return p, X, T
+
+
+def test_forward_matches_oracle(cm):
+    p, X, _ = _setup(1)
+    net = cm.MLP()
+    net.set_params(p)
+    y = net.forward_raw(X)
+    want, _ = mo.forward(p.astype(np.float64), X.astype(np.float64))
+    assert np.allclose(y, want, rtol=RTOL, atol=ATOL), np.abs(y - want).max()
+
+
+def test_gradients_match_oracle(cm):
+    p, X, T = _setup(2)
+    net = cm.MLP()
+    net.set_params(p)
+    g, loss = net.gradients_raw(X, T)
+    want, wl = mo.gradients(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - wl) <= 1e-4 * abs(wl) + 1e-6
+    scale = np.abs(want).max()
+    assert np.allclose(g, want, rtol=RTOL, atol=ATOL * scale), np.abs(g - want).max()
+
+
+def test_adam_matches_oracle(cm):
+    p, X, T = _setup(3)
+    net = cm.MLP()
+    net.set_params(p)
+    rng = np.random.default_rng(5)
+    pp = p.astype(np.float64)
+    m = np.zeros(mo.NPARAMS)
+    v = np.zeros(mo.NPARAMS)
+    for t in range(1, 4):
+        g = rng.normal(0, 1e-2, mo.NPARAMS).astype(np.float32)
+        net.adam_step(g, 1e-3)
+        pp, m, v = mo.adam(pp, g.astype(np.float64), m, v, t, 1e-3)
+    got = net.get_params()
+    assert np.allclose(got, pp, rtol=1e-5, atol=1e-6), np.abs(got - pp).max()
+
+
+def test_training_lowers_loss(cm):
+    rng = np.random.default_rng(6)
+    X = rng.normal(0, 1, (4096, 6))
+    y = np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 3] + 0.1 * X[:, 4]
+    net = cm.MLP(seed=1)
+    losses = net.fit(X, y, epochs=30, batch=128, lr=3e-3, seed=2)
+    assert losses[-1] < 0.1 * losses[0], losses
+
+
+@pytest.mark.skipif(not os.path.exists(DATA), reason="timing dataset not collected")
+def test_tuner_regret_on_held_out_queries(cm):
+    """Train on the training queries' timings, choose a geometry for each of
+    the held-out queries, and compare with grid search (the measured best) and
+    with the fixed default."""
+    d = np.load(DATA, allow_pickle=False)
+    qtr, ttr, qte, tte = d["q_train"], d["t_train"], d["q_test"], d["t_test"]
+    grid = [tuple(g) for g in d["grid"]]
+    qtr = [tuple(int(v) for v in q) for q in qtr]
+    qte = [tuple(int(v) for v in q) for q in qte]
+    net = cm.MLP(seed=0)
+    net.fit(cm.design_matrix(qtr, grid), np.log2(ttr.reshape(-1)), epochs=60, batch=256, lr=2e-3, seed=0)
+    chosen, _ = cm.choose(net, qte, grid)
+    r = cm.regret(chosen, tte, grid.index(cm.DEFAULT))
+    print("regret", r, flush=True)
+    assert r["total_regret"] < r["default_total_regret"]
+    assert r["gain_over_default"] > 1.0
