@@ -1,38 +1,54 @@
 """Benchmark of the gfx950 counting backend on one MI355X.
 
-Workload (a "step"): the single large interval [1e13, 1e13 + 1e11] -- all six
-counts (primes, twins, triplets, quadruplets, quintuplets, sextuplets) from
-one sieve pass, through the C ABI (psg_gpu_count_all).  The sieving primes
-(<= isqrt(1.01e13)) are generated and uploaded during warm-up, so the timed
-steps start with every input resident in HBM.
+Headline workload (a "step"): the single large interval [1e13, 1e13 + 1e11]
+-- all six counts (primes, twins, triplets, quadruplets, quintuplets,
+sextuplets) from one sieve pass, through the C ABI (psg_gpu_count_all).  The
+sieving primes (<= isqrt(1.01e13)) are generated and uploaded during warm-up,
+so the timed steps start with every input resident in HBM.  Every count of
+every step is checked against the reference's values.
 
 Prints ONE JSON line:
-  value      integers sieved per second, (stop - start + 1) * steps / seconds
-  roofline   the segment kernel against the measured LDS roofline:
-             achieved = algorithmic LDS bytes per launch / average launch
-             duration (HIP events on the backend's stream);
-             algorithmic LDS bytes = 8 * C + 8 * W, C = crossings by sieving
-             primes >= 29, W = 32-bit words of the bit array (4 B fill write +
-             4 B count read each); peak = the same 8-byte-per-atomic figure
-             measured by the LDS ds_and_b32 microbenchmark
+  value         integers sieved per second, (stop - start + 1) * steps / seconds
+  roofline      the segment kernel (psg_sieve_count) against the measured LDS
+                roofline: achieved = algorithmic LDS bytes per launch / average
+                launch duration (HIP events on the backend's stream), with
+                algorithmic LDS bytes = 8*C + 8*W (C = crossings by sieving
+                primes >= 29: one 4-byte read + one 4-byte write each; W =
+                32-bit words of the bit array: one fill write + one count
+                read); peak = the same 8-bytes-per-ds_and_b32 figure measured
+                by the conflict-free LDS microbenchmark on the same device;
+                traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE /
+                WRITE_SIZE summary in profiles/ when present
   cpu_baseline  the reference CPU sieve (oracle/_ref, built from the
-             reference sources) on the host cores, counting primes over the
-             same interval
-Every count of every step is checked against the reference's golden values.
+                reference sources) on the host cores, counting primes over the
+                same interval
+  workloads     the other configurations: pi(1e10); 8192 uneven intervals
+                batched vs one call per interval (intervals/s), with and
+                without the cost model; the cost model's training and regret;
+                the Poisson serving loop
 """
 import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(ROOT, "primesieve-gpu_amd"))
+PKG = os.path.join(ROOT, "primesieve-gpu_amd")
+sys.path.insert(0, PKG)
 
 START, STOP = 10 ** 13, 10 ** 13 + 10 ** 11
 GOLDEN = [3340141707, 147308319, 21308636, 517256, 84202, 2390]   # reference CPU sieve
-HBM_PEAK_GBS = 8000.0
+PI_1E10 = 455052511
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "psg_sieve_count_interval_1e13_pmc.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def cpu_baseline():
@@ -46,35 +62,146 @@ def cpu_baseline():
         L.primesieve_count_primes.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.primesieve_set_num_threads.argtypes = [ctypes.c_int]
         L.primesieve_set_num_threads(threads)
-        a, b = START, STOP
         t0 = time.perf_counter()
-        n = L.primesieve_count_primes(a, b)
+        n = L.primesieve_count_primes(START, STOP)
         dt = time.perf_counter() - t0
         if n != GOLDEN[0]:
             raise RuntimeError("reference CPU count mismatch: %d" % n)
-        return {"value": (b - a + 1) / dt, "unit": "integers/s", "cores": threads,
+        return {"value": (STOP - START + 1) / dt, "unit": "integers/s", "cores": threads,
                 "kind": "reference",
-                "sample": "primesieve_count_primes over [1e13, 1e13+1e11], %d threads, %.2f s"
-                          % (threads, dt)}
-    # fall back to the oracle restatement on a bounded sample
+                "sample": "primesieve_count_primes over [1e13, 1e13+1e11] (the whole interval), "
+                          "%d threads, %.2f s" % (threads, dt)}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from conftest import Oracle
-    o = Oracle()
     a, b = START, START + 10 ** 10
     t0 = time.perf_counter()
-    o.count6(a, b, threads)
+    Oracle().count6(a, b, threads)
     dt = time.perf_counter() - t0
     return {"value": (b - a + 1) / dt, "unit": "integers/s", "cores": threads, "kind": "port",
             "sample": "oracle count6 over [1e13, 1e13+1e10], %d threads, %.2f s" % (threads, dt)}
 
 
-def lds_peak_gbs(m):
-    """Measured LDS roofline in the 8-bytes-per-ds_and convention (GB/s)."""
+def profiled_traffic():
+    """HBM bytes per psg_sieve_count launch from the committed PMC summary."""
     try:
-        r = m.lds_roofline()
-        return r["and_conflict_free_gbs"], r
-    except Exception as e:  # microbenchmark not available
-        return None, {"error": str(e)}
+        text = open(TRAFFIC_PROFILE).read()
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def headline(m, steps, warmup):
+    for _ in range(max(1, warmup)):
+        if m.count_all(START, STOP) != GOLDEN:
+            raise SystemExit("count mismatch in warm-up")
+    launch_ms, launches = [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = m.count_all(START, STOP)
+        st = m.last_stats()
+        launch_ms.append(st.kernel_ms / max(1, st.launches))
+        launches = st.launches
+        if got != GOLDEN:
+            raise SystemExit("count mismatch: %s" % got)
+    dt = time.perf_counter() - t0
+    return got, dt, m.last_stats(), sum(launch_ms) / len(launch_ms), launches
+
+
+def workload_pi(m):
+    t0 = time.perf_counter()
+    for _ in range(3):
+        n = m.count(0, 10 ** 10, 0)
+        if n != PI_1E10:
+            raise SystemExit("pi(1e10) mismatch: %d" % n)
+    dt = (time.perf_counter() - t0) / 3
+    return {"workload": "pi(1e10), primes in [0, 1e10]", "count": n, "ms": dt * 1e3,
+            "integers_per_s": (10 ** 10 + 1) / dt, "device_ms": m.last_stats().device_ms}
+
+
+def workload_batch(m, model):
+    q = np.load(os.path.join(ROOT, "tests", "golden", "config3_queries.npy"), allow_pickle=False)
+    qs = [(int(a), int(b), int(k)) for a, b, k in q]
+    m.count_all(10 ** 18, 10 ** 18 + 10 ** 6)      # sieving primes up to 1e9 cached (untimed)
+    res = {"workload": "8192 uneven intervals, start 1e9..1e18, length 2^16..2^32, k 1..6"}
+    t0 = time.perf_counter()
+    batched = m.count_batch(qs)
+    res["batched_s"] = time.perf_counter() - t0
+    res["batched_stats"] = {k: v for k, v in m.last_stats().as_dict().items()
+                            if k in ("kernel_ms", "fill_ms", "device_ms", "launches", "segments")}
+    t0 = time.perf_counter()
+    plain = m.count_batch(qs, mode=m.BATCH_PLAIN)
+    res["plain_s"] = time.perf_counter() - t0
+    if plain != batched:
+        raise SystemExit("batched and plain counts differ")
+    exp = json.load(open(os.path.join(ROOT, "tests", "golden", "config3_expected.json")))
+    bad = [i for i, v in exp.items() if batched[int(i)] != v]
+    if bad:
+        raise SystemExit("batched counts differ from the reference on %d queries" % len(bad))
+    res["checked_against_reference"] = len(exp)
+    res["batched_intervals_per_s"] = len(qs) / res["batched_s"]
+    res["plain_intervals_per_s"] = len(qs) / res["plain_s"]
+    res["speedup"] = res["plain_s"] / res["batched_s"]
+    if model is not None:
+        import costmodel
+        t0 = time.perf_counter()
+        (sb, waves), order = costmodel.choose_for_batch(
+            model, [(a, b - a + 1, k) for a, b, k in qs])
+        m.set_tuning(sieve_bytes=sb, work_group_size=64 * waves)
+        m.count_all(10 ** 18, 10 ** 18 + 10 ** 6)
+        plan_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tuned = m.count_batch([qs[i] for i in order])
+        res["model_batched_s"] = time.perf_counter() - t0
+        m.set_tuning()
+        back = [0] * len(qs)
+        for j, i in enumerate(order):
+            back[i] = tuned[j]
+        if back != batched:
+            raise SystemExit("model-packed counts differ")
+        res["model_choice"] = {"sieve_bytes": sb, "waves": waves, "plan_s": plan_s}
+        res["model_batched_intervals_per_s"] = len(qs) / res["model_batched_s"]
+        res["model_speedup_vs_plain"] = res["plain_s"] / res["model_batched_s"]
+    return res
+
+
+def workload_costmodel():
+    path = os.path.join(PKG, "data", "costmodel_timings.npz")
+    if not os.path.exists(path):
+        return None, None
+    import costmodel
+    d = np.load(path, allow_pickle=False)
+    grid = [tuple(int(v) for v in g) for g in d["grid"]]
+    qtr = [tuple(int(v) for v in q) for q in d["q_train"]]
+    qte = [tuple(int(v) for v in q) for q in d["q_test"]]
+    X = costmodel.design_matrix(qtr, grid)
+    y = np.log2(d["t_train"].reshape(-1))
+    net = costmodel.MLP(seed=0)
+    t0 = time.perf_counter()
+    losses = net.fit(X, y, epochs=60, batch=256, lr=2e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    chosen, pred = costmodel.choose(net, qte, grid)
+    r = costmodel.regret(chosen, d["t_test"], grid.index(costmodel.DEFAULT))
+    test_mse = float(np.mean((pred.reshape(-1) - np.log2(d["t_test"].reshape(-1))) ** 2))
+    return net, {"workload": "MLP 6-64-64-1 FP32, %d training timings, %d held-out queries x %d geometries"
+                 % (X.shape[0], len(qte), len(grid)),
+                 "train_s": train_s, "epochs": len(losses), "final_train_loss": float(losses[-1]),
+                 "heldout_mse_log2": test_mse, **r}
+
+
+def workload_serving(m, model, seconds=4.0, qps=2000.0):
+    import serve
+    n = int(seconds * qps)
+    arrivals, qs = serve.poisson_workload(n, qps, seed=5)
+    m.count_all(10 ** 18, 10 ** 18 + 10 ** 6)
+    counts, lat, wall, batches = serve.serve(arrivals, qs, model=model)
+    check = m.count_batch(qs, mode=m.BATCH_PLAIN)
+    if check != counts:
+        raise SystemExit("served counts differ from the plain path")
+    s = serve.summary(lat, wall, n, batches)
+    s["workload"] = "Poisson %.0f QPS for %.0f s, Pareto(1.2) lengths in [2^12, 2^36], k 1..6" % (qps, seconds)
+    s["counts_checked"] = "all %d against the plain per-query path" % n
+    return s
 
 
 def main():
@@ -83,46 +210,31 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the count path runs on a single MI355X "
-              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        print("bench.py measures one GPU: the count path runs on a single MI355X; "
+              "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
         sys.exit(2)
 
     import psgpu as m
     if not m.available():
         raise SystemExit("no usable GPU: " + m.last_error())
 
-    for _ in range(max(1, args.warmup)):
-        got = m.count_all(START, STOP)
-        if got != GOLDEN:
-            raise SystemExit("count mismatch in warm-up: %s" % got)
-
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        got = m.count_all(START, STOP)
-        st = m.last_stats()
-        kernel_ms.append(st.kernel_ms / max(1, st.launches))
-        if got != GOLDEN:
-            raise SystemExit("count mismatch: %s" % got)
-    dt = time.perf_counter() - t0
-
-    n_int = (STOP - START + 1) * args.steps
-    value = n_int / dt
-    st = m.last_stats()
+    got, dt, st, avg_launch_ms, launches = headline(m, args.steps, args.warmup)
+    value = (STOP - START + 1) * args.steps / dt
     work = m.algorithmic_work(START, STOP, st.sieve_bytes)
-    lds_bytes_launch = (8 * work.crossings + 8 * work.words32) / max(1, st.launches)
-    avg_launch_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    achieved = lds_bytes_launch / avg_launch_s / 1e9
-    peak, lds_info = lds_peak_gbs(m)
+    lds_bytes_launch = (8 * work.crossings + 8 * work.words32) / max(1, launches)
+    achieved = lds_bytes_launch / (avg_launch_ms / 1e3) / 1e9
+    lds = m.lds_roofline()
+    peak = lds["and_conflict_free_gbs"]
     roofline = {"bound": "lds", "achieved": achieved, "peak": peak, "unit": "GB/s",
-                "frac": (achieved / peak) if peak else None, "traffic": None,
-                "kernel": "psg_sieve_count", "avg_launch_ms": avg_launch_s * 1e3,
-                "launches_per_step": st.launches,
-                "lds_bytes_per_launch": lds_bytes_launch,
-                "crossings": work.crossings, "words32": work.words32}
-
+                "frac": achieved / peak, "traffic": profiled_traffic(),
+                "kernel": "psg_sieve_count", "avg_launch_ms": avg_launch_ms,
+                "launches_per_step": launches, "lds_bytes_per_launch": lds_bytes_launch,
+                "crossings": work.crossings, "words32": work.words32,
+                "peak_source": "measured ds_and_b32 microbenchmark (psg_lds_and, conflict-free)",
+                "measured": lds}
     out = {
         "metric": "integers sieved/s on 1 MI355X, exact counts",
         "value": value,
@@ -138,12 +250,23 @@ def main():
         "data": "synthetic (fixed interval)",
         "config": {"workload": "single interval [1e13, 1e13+1e11], six counts in one pass",
                    "sieve_bytes": st.sieve_bytes, "work_group_size": st.work_group_size,
-                   "sieving_primes": st.sieving_primes, "segments": st.segments},
+                   "sieving_primes": st.sieving_primes, "segments": st.segments,
+                   "segment_kernel_ms": st.kernel_ms, "bucket_kernels_ms": st.fill_ms},
         "counts": got,
         "roofline": roofline,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
-        "lds_microbench": lds_info,
     }
+    if not args.headline_only:
+        wl = {}
+        log("workload: pi(1e10)")
+        wl["pi_1e10"] = workload_pi(m)
+        log("workload: cost model")
+        model, wl["cost_model"] = workload_costmodel()
+        log("workload: batched intervals")
+        wl["batched_intervals"] = workload_batch(m, model)
+        log("workload: serving")
+        wl["serving"] = workload_serving(m, model)
+        out["workloads"] = wl
     print(json.dumps(out), flush=True)
 
 

@@ -61,8 +61,12 @@ def collect(queries, grid=GRID, reps=1, log=None):
     """Time one count call per (query, tuning) on the GPU.  Returns an
     (n_queries, len(grid)) array of device milliseconds (min over reps)."""
     times = np.zeros((len(queries), len(grid)), dtype=np.float64)
+    top = max(queries, key=lambda q: q[0] + q[1])
     for gi, (sb, waves) in enumerate(grid):
         psgpu.set_tuning(sieve_bytes=sb, work_group_size=64 * waves)
+        # a new geometry is a new backend: fill its sieving-prime cache once,
+        # up to the largest stop, so that no timed call regenerates it
+        psgpu.count(top[0], top[0] + 29, 0)
         t0 = time.perf_counter()
         for qi, (a, length, k) in enumerate(queries):
             best = float("inf")
@@ -70,6 +74,8 @@ def collect(queries, grid=GRID, reps=1, log=None):
                 psgpu.count(a, a + length - 1, k - 1)
                 best = min(best, psgpu.last_stats().device_ms)
             times[qi, gi] = best
+            if log and qi % 1000 == 999:
+                log("config %s: %d queries, %.1f s" % ((sb, waves), qi + 1, time.perf_counter() - t0))
         if log:
             log("config %s: %d queries in %.1f s" % ((sb, waves), len(queries), time.perf_counter() - t0))
     psgpu.set_tuning()

@@ -180,7 +180,24 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     uint32_t i1 = (uint32_t) ((B + 4ull * tid) % PSG_TILE1_PERIOD);
     const uint32_t s0 = (4u * nthreads) % PSG_TILE0_PERIOD;
     const uint32_t s1 = (4u * nthreads) % PSG_TILE1_PERIOD;
-    for (uint32_t w = tid; w < words; w += nthreads) {
+    // eight words per trip: all sixteen tile loads are issued before the
+    // first LDS store waits on them
+    uint32_t w = tid;
+    for (; w + 7u * nthreads < words; w += 8u * nthreads) {
+      uint32_t a0[8], a1[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        a0[u] = i0; a1[u] = i1;
+        i0 += s0; if (i0 >= PSG_TILE0_PERIOD) i0 -= PSG_TILE0_PERIOD;
+        i1 += s1; if (i1 >= PSG_TILE1_PERIOD) i1 -= PSG_TILE1_PERIOD;
+      }
+      uint32_t v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = tile0[a0[u]] & tile1[a1[u]];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sieve[w + u * nthreads] = v[u];
+    }
+    for (; w < words; w += nthreads) {
       sieve[w] = tile0[i0] & tile1[i1];
       i0 += s0; if (i0 >= PSG_TILE0_PERIOD) i0 -= PSG_TILE0_PERIOD;
       i1 += s1; if (i1 >= PSG_TILE1_PERIOD) i1 -= PSG_TILE1_PERIOD;
@@ -218,11 +235,15 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         const uint32_t k = b + slot;
         const uint32_t p = (k < t0) ? primes[k] : 0u;
         const uint32_t st = p ? class_start(p, mod_magic(Bs, p, magics[k]), Bs, j, ctab) : 0u;
+        // all eight permutes first: one wait for the batch, not one per prime
+        uint32_t sq[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) sq[q] = (uint32_t) __shfl((int) st, (int) (q * 8u + j));
+#pragma unroll
         for (uint32_t q = 0; q < 8; q++) {
           const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
           if (!ps) break;
-          const uint32_t pos = sliceLo + (uint32_t) __shfl((int) st, (int) (q * 8u + j)) + slot * ps;
-          strike_run(sieve, pos, 8u * ps, sliceHi, j);
+          strike_run(sieve, sliceLo + sq[q] + slot * ps, 8u * ps, sliceHi, j);
         }
       }
     }
@@ -244,11 +265,14 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         pN = (bn < nb && kN < t1) ? primes[kN] : 0u;
         MN = (bn < nb && kN < t1) ? magics[kN] : 0ull;
         const uint32_t st = p ? class_start(p, mod_magic(B, p, M), B, j, ctab) : 0u;
+        uint32_t sq[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) sq[q] = (uint32_t) __shfl((int) st, (int) (q * 8u + j));
+#pragma unroll
         for (uint32_t q = 0; q < 8; q++) {
           const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
           if (!ps) break;
-          const uint32_t pos = (uint32_t) __shfl((int) st, (int) (q * 8u + j)) + slot * ps;
-          strike_run(sieve, pos, 8u * ps, S, j);
+          strike_run(sieve, sq[q] + slot * ps, 8u * ps, S, j);
         }
       }
     }
