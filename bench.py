@@ -251,6 +251,7 @@ def main():
         "config": {"workload": "single interval [1e13, 1e13+1e11], six counts in one pass",
                    "sieve_bytes": st.sieve_bytes, "work_group_size": st.work_group_size,
                    "sieving_primes": st.sieving_primes, "segments": st.segments,
+                   "bucket_entries": st.bucket_entries,
                    "segment_kernel_ms": st.kernel_ms, "bucket_kernels_ms": st.fill_ms},
         "counts": got,
         "roofline": roofline,
