@@ -55,6 +55,8 @@ typedef struct psg_tuning {
   uint32_t slab_segments;     /* segments per big-prime bucket slab */
   uint32_t bucket_factor;     /* primes > bucket_factor * sieve_bytes use buckets */
   uint32_t presieve;          /* 1 = pre-sieve tiles for 7..23 */
+  uint32_t bucket_margin;     /* room of a segment's bucket list beyond the expected
+                                 count, in standard deviations; 0 = auto (6) */
 } psg_tuning;
 int psg_set_gpu_tuning(const psg_tuning* t);
 int psg_get_gpu_tuning(psg_tuning* t);
@@ -120,6 +122,8 @@ typedef struct psg_stats {
   uint32_t sieve_bytes;      /* LDS bytes per segment */
   uint32_t work_group_size;  /* threads per workgroup */
   uint32_t sieving_primes;   /* primes >= 29 with p*p <= stop */
+  uint32_t bucket_retries;   /* calls repeated because a bucket list outgrew its room */
+  uint32_t reserved;
 } psg_stats;
 int psg_gpu_last_stats(psg_stats* out);
 

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -68,11 +69,10 @@ HipBackend::~HipBackend()
   counts_.release();
   descriptors_.release();
   workCounters_.release();
-  bucketCounts_.release();
-  listOff_.release();
+  listCount_.release();
   entries_.release();
-  overflow_.release();
-  if (hostOverflow_) (void) api_->HostFree(hostOverflow_);
+  flags_.release();
+  if (hostFlags_) (void) api_->HostFree(hostFlags_);
   for (hipEvent_t e : phaseEvents_) (void) api_->EventDestroy(e);
   if (evStart_) (void) api_->EventDestroy(evStart_);
   if (evStop_) (void) api_->EventDestroy(evStop_);
@@ -110,15 +110,14 @@ void HipBackend::init()
                                        : (uint32_t) ((128ull << 20) / sieveBytes_);
   slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 8192));
   windowSegs_ = 64;
+  bucketMargin_ = config_.bucketMargin ? config_.bucketMargin : 6u;
   firstCrossPrime_ = config_.preSieve ? PSG_FIRST_CROSS_PRIME : 7u;
 
   const size_t coBytes = (size_t) (psg_sieve_hsaco_end - psg_sieve_hsaco_start);
   hipCheck(api_, api_->ModuleLoadData(&module_, psg_sieve_hsaco_start),
            ("loading the gfx950 code object (" + std::to_string(coBytes) + " bytes)").c_str());
   const struct { hipFunction_t* f; const char* name; } fns[] = {
-    { &fnSieve_, "psg_sieve_count" }, { &fnBucketCount_, "psg_bucket_count" },
-    { &fnScanChunks_, "psg_bucket_scan_chunks" }, { &fnScanLists_, "psg_bucket_scan_lists" },
-    { &fnBucketFill_, "psg_bucket_fill" } };
+    { &fnSieve_, "psg_sieve_count" }, { &fnBucketFill_, "psg_bucket_fill" } };
   for (const auto& f : fns)
     hipCheck(api_, api_->ModuleGetFunction(f.f, module_, f.name),
              (std::string("hipModuleGetFunction(") + f.name + ")").c_str());
@@ -138,9 +137,9 @@ void HipBackend::init()
     hipCheck(api_, api_->Memcpy(tile1_.get(), t1.data(), t1.size() * 4, hipMemcpyHostToDevice), "hipMemcpy tile1");
   }
   void* hc = nullptr;
-  hipCheck(api_, api_->HostMalloc(&hc, 4, 0), "hipHostMalloc");
-  hostOverflow_ = (uint32_t*) hc;
-  *hostOverflow_ = 0;
+  hipCheck(api_, api_->HostMalloc(&hc, 16, 0), "hipHostMalloc");
+  hostFlags_ = (uint64_t*) hc;
+  hostFlags_[0] = hostFlags_[1] = 0;
 }
 
 void HipBackend::ensureSievingPrimes(uint32_t limit)
@@ -201,7 +200,12 @@ uint32_t HipBackend::persistentGrid(uint32_t nSegs)
 /// Sieve every interval; out6[slot * 6 + t] receives the six counts of the
 /// intervals whose `out` is slot.  Intervals are cut into runs of at most
 /// opt.runSegs segments, runs are packed into slabs, and each slab is one
-/// bucket build (when big primes exist) plus one persistent sieve launch.
+/// bucket fill (when big primes exist) plus one persistent sieve launch.
+///
+/// Every segment's bucket list gets a room of E + m*sqrt(E) + 64 entries,
+/// E = 8 * S * sum(1/p) over its big primes (the expected crossings).  The
+/// fill counts what it appends; if any list outgrew its room, the counted
+/// totals become the rooms and the call runs once more.
 void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6,
                          const ExecOptions& opt)
 {
@@ -223,12 +227,12 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
   uint32_t shift = 0;
   while ((1u << shift) < S) shift++;
   const bool pow2 = (1u << shift) == S;
-  const uint32_t bigBegin = (pow2 && bucketFactor_)
-                          ? primesUpTo((uint64_t) S * bucketFactor_) : (uint32_t) hostPrimes_.size();
-  const uint32_t entryBytes = (S <= 65536u) ? 2u : 4u;
+  bigBegin_ = (pow2 && bucketFactor_)
+            ? primesUpTo((uint64_t) S * bucketFactor_) : (uint32_t) hostPrimes_.size();
+  const uint32_t windowSegs = std::max<uint32_t>(1, std::min(opt.windowSegs, PSG_MAX_WINDOW_SEGS));
 
   // ---- runs -----------------------------------------------------------------
-  std::vector<PsgRun> runs;
+  runs_.clear();
   uint64_t segsTotal = 0;
   for (const Interval& v : iv)
   {
@@ -250,99 +254,137 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
       r.nprimes = np;
       r.out = v.out;
       r.nseg = (uint32_t) std::min<uint64_t>(opt.runSegs, nseg - s);
-      runs.push_back(r);
+      runs_.push_back(r);
     }
     segsTotal += nseg;
   }
-  if (runs.empty())
+  if (runs_.empty())
     return;
 
-  // ---- slabs and windows ------------------------------------------------------
-  struct Slab { size_t run0, run1, win0, win1; uint32_t nSegs, nWGs, countsWords, capacity, maxLists; };
-  std::vector<Slab> slabs;
-  std::vector<PsgWindow> wins;
-  const double hBig = invPrefix_[bigBegin];
-  const auto expectedHits = [&](uint32_t bytes, uint32_t bigEnd) {
-    return 8.0 * bytes * (invPrefix_[bigEnd] - hBig);
-  };
+  // ---- slabs, windows and the expected bucket entries per segment ------------
+  slabs_.clear();
+  wins_.clear();
+  segExpect_.clear();
   {
-    Slab cur{ 0, 0, 0, 0, 0, 0, 0, 0, 0 };
-    double est = 0;
-    for (size_t i = 0; i < runs.size(); i++)
+    const double hBig = invPrefix_[bigBegin_];
+    Slab cur{ 0, 0, 0, 0, 0, 0, 0, 0.0 };
+    for (size_t i = 0; i < runs_.size(); i++)
     {
-      PsgRun& r = runs[i];
-      const bool big = r.nprimes > bigBegin;
-      const double e = big ? expectedHits(r.nseg * S, r.nprimes) : 0.0;
+      PsgRun& r = runs_[i];
+      const bool big = r.nprimes > bigBegin_;
+      const double perSeg = big ? 8.0 * S * (invPrefix_[r.nprimes] - hBig) : 0.0;
+      const double e = perSeg * r.nseg;
       if (cur.run1 > cur.run0 &&
-          (cur.nSegs + r.nseg > opt.slabSegs || est + e > opt.entryBudget))
+          (cur.nSegs + r.nseg > opt.slabSegs || cur.est + e > opt.entryBudget))
       {
-        cur.win1 = wins.size();
-        cur.capacity = (uint32_t) std::min(1.25 * est + 65536.0 + 64.0 * (cur.win1 - cur.win0), 2.0e9);
-        slabs.push_back(cur);
-        cur = Slab{ i, i, wins.size(), wins.size(), 0, 0, 0, 0, 0 };
-        est = 0;
+        cur.win1 = wins_.size();
+        slabs_.push_back(cur);
+        cur = Slab{ i, i, wins_.size(), wins_.size(), 0, 0, cur.segBase + cur.nSegs, 0.0 };
       }
       r.segStart = cur.nSegs;
       if (big)
       {
-        const uint32_t nBig = r.nprimes - bigBegin;
-        for (uint32_t s = 0; s < r.nseg; s += opt.windowSegs)
+        const uint32_t nBig = r.nprimes - bigBegin_;
+        const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
+            opt.maxChunks, (nBig + opt.primesPerChunk - 1) / opt.primesPerChunk));
+        for (uint32_t s = 0; s < r.nseg; s += windowSegs)
         {
           PsgWindow w;
-          w.segs = std::min(opt.windowSegs, r.nseg - s);
-          // enough chunks to spread the primes, but a bounded counts block
-          const uint32_t cap = std::max(opt.maxChunks, opt.chunkListBudget / (8u * w.segs));
-          const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
-              cap, (nBig + opt.primesPerChunk - 1) / opt.primesPerChunk));
+          std::memset(&w, 0, sizeof w);
+          w.segs = std::min(windowSegs, r.nseg - s);
           w.base = r.B0 + (uint64_t) s * S;
-          w.firstList = 8u * (cur.nSegs + s);
+          w.firstSeg = cur.nSegs + s;
           w.bigEnd = r.nprimes;
           w.nChunks = chunks;
-          w.countsBase = cur.countsWords;
           w.wgBase = cur.nWGs;
-          cur.countsWords += chunks * 8u * w.segs;
           cur.nWGs += chunks;
-          cur.maxLists = std::max(cur.maxLists, 8u * w.segs);
-          wins.push_back(w);
+          wins_.push_back(w);
         }
       }
-      est += e;
+      segExpect_.insert(segExpect_.end(), r.nseg, perSeg);
+      cur.est += e;
       cur.nSegs += r.nseg;
       cur.run1 = i + 1;
     }
-    cur.win1 = wins.size();
-    cur.capacity = (uint32_t) std::min(1.25 * est + 65536.0 + 64.0 * (cur.win1 - cur.win0), 2.0e9);
-    slabs.push_back(cur);
+    cur.win1 = wins_.size();
+    slabs_.push_back(cur);
+  }
+
+  std::vector<uint32_t> room(segExpect_.size());
+  for (size_t s = 0; s < room.size(); s++)
+  {
+    const double E = segExpect_[s];
+    room[s] = E > 0 ? (uint32_t) std::min(std::ceil(E + bucketMargin_ * std::sqrt(E) + 64.0), 4.0e9) : 0u;
+  }
+  if (!launchPlan(outs, room))
+  {
+    // a list outgrew its room: the fill counted every list exactly
+    hipCheck(api_, api_->Memcpy(room.data(), listCount_.get(), room.size() * 4, hipMemcpyDeviceToHost),
+             "hipMemcpy bucket list counts");
+    stats_.bucket_retries = 1;
+    if (!launchPlan(outs, room))
+      throw primesieve_error("big-prime bucket list outgrew its counted room");
+  }
+  std::copy(hostCounts_.begin(), hostCounts_.end(), out6);
+  stats_.segments = segsTotal;
+  stats_.launches = (uint32_t) slabs_.size();
+  stats_.sieving_primes = primesUpTo(isqrt64(maxHi));
+  stats_.wall_ms = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t0).count();
+}
+
+/// Launch the current plan with the given bucket list rooms (one per
+/// segment of the call).  False when a list outgrew its room (the counts are
+/// then incomplete and listCount_ holds the exact list sizes).
+bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
+{
+  const uint32_t S = sieveBytes_;
+  uint32_t shift = 0;
+  while ((1u << shift) < S) shift++;
+
+  // per-slab list offsets, and the largest slab's entries
+  std::vector<uint32_t> listOff;
+  std::vector<size_t> offPos(slabs_.size());
+  uint64_t maxEntries = 0;
+  for (size_t si = 0; si < slabs_.size(); si++)
+  {
+    const Slab& sl = slabs_[si];
+    offPos[si] = listOff.size();
+    uint64_t acc = 0;
+    for (uint32_t s = 0; s < sl.nSegs; s++)
+    {
+      listOff.push_back((uint32_t) acc);
+      acc += room[sl.segBase + s];
+    }
+    if (acc > 0xffffffffull)
+      throw primesieve_error("big-prime bucket slab exceeds 2^32 entries");
+    listOff.push_back((uint32_t) acc);
+    maxEntries = std::max(maxEntries, acc);
   }
 
   // ---- device buffers and one upload of every descriptor ------------------------
-  uint32_t maxSegs = 0, maxCounts = 0, maxCap = 0;
-  for (const Slab& sl : slabs)
-  {
-    maxSegs = std::max(maxSegs, sl.nSegs);
-    maxCounts = std::max(maxCounts, sl.countsWords);
-    maxCap = std::max(maxCap, sl.capacity);
-  }
-  const size_t runBytes = runs.size() * sizeof(PsgRun);
-  const size_t winBytes = wins.size() * sizeof(PsgWindow);
-  descriptors_.allocate(api_, runBytes + winBytes + 16);
-  staging_.resize(runBytes + winBytes);
-  std::memcpy(staging_.data(), runs.data(), runBytes);
-  if (winBytes) std::memcpy(staging_.data() + runBytes, wins.data(), winBytes);
-  hipCheck(api_, api_->MemcpyAsync(descriptors_.get(), staging_.data(), runBytes + winBytes,
+  const size_t runBytes = runs_.size() * sizeof(PsgRun);
+  const size_t winBytes = wins_.size() * sizeof(PsgWindow);
+  const size_t offBytes = listOff.size() * 4;
+  const size_t nSegsAll = room.size();
+  descriptors_.allocate(api_, runBytes + winBytes + offBytes + 16);
+  staging_.resize(runBytes + winBytes + offBytes);
+  std::memcpy(staging_.data(), runs_.data(), runBytes);
+  if (winBytes) std::memcpy(staging_.data() + runBytes, wins_.data(), winBytes);
+  std::memcpy(staging_.data() + runBytes + winBytes, listOff.data(), offBytes);
+  hipCheck(api_, api_->MemcpyAsync(descriptors_.get(), staging_.data(), staging_.size(),
                                    hipMemcpyHostToDevice, stream_), "hipMemcpyAsync descriptors");
+  const uint64_t offBase = descriptors_.addr() + runBytes + winBytes;
   counts_.allocate(api_, (size_t) outs * PSG_NCOUNTS * sizeof(uint64_t));
-  workCounters_.allocate(api_, slabs.size() * 4);
-  listOff_.allocate(api_, ((size_t) 8 * maxSegs + 1) * 4);
-  if (maxCounts)
-  {
-    bucketCounts_.allocate(api_, (size_t) maxCounts * 4);
-    entries_.allocate(api_, (size_t) maxCap * entryBytes);
-  }
-  overflow_.allocate(api_, 4);
+  workCounters_.allocate(api_, slabs_.size() * 4);
+  listCount_.allocate(api_, nSegsAll * 4 + 4);
+  if (maxEntries)
+    entries_.allocate(api_, (size_t) maxEntries * 4);
+  flags_.allocate(api_, 16);
   hipCheck(api_, api_->MemsetAsync(counts_.get(), 0, (size_t) outs * PSG_NCOUNTS * 8, stream_), "hipMemsetAsync");
-  hipCheck(api_, api_->MemsetAsync(workCounters_.get(), 0, slabs.size() * 4, stream_), "hipMemsetAsync");
-  hipCheck(api_, api_->MemsetAsync(overflow_.get(), 0, 4, stream_), "hipMemsetAsync");
+  hipCheck(api_, api_->MemsetAsync(workCounters_.get(), 0, slabs_.size() * 4, stream_), "hipMemsetAsync");
+  hipCheck(api_, api_->MemsetAsync(listCount_.get(), 0, nSegsAll * 4, stream_), "hipMemsetAsync");
+  hipCheck(api_, api_->MemsetAsync(flags_.get(), 0, 16, stream_), "hipMemsetAsync");
 
   PsgSieveArgs args;
   std::memset(&args, 0, sizeof args);
@@ -351,62 +393,50 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
   args.tile0 = tile0_.addr();
   args.tile1 = tile1_.addr();
   args.counts = counts_.addr();
-  args.listOff = listOff_.addr();
   args.segBytes = S;
   args.tier0End = primesUpTo(std::min<uint32_t>(PSG_TIER0_MAX_PRIME, S / 8));
   args.tier1End = primesUpTo(S / 8);
   args.tier2End = primesUpTo(S);
-  args.tier3End = bigBegin;
-  args.entryBytes = entryBytes;
+  args.tier3End = bigBegin_;
 
   PsgBucketArgs b;
   std::memset(&b, 0, sizeof b);
   b.primes = primes_.addr();
   b.magics = magics_.addr();
-  b.counts = bucketCounts_.addr();
-  b.listOff = listOff_.addr();
   b.entries = entries_.addr();
-  b.overflow = overflow_.addr();
+  b.overflow = flags_.addr();
+  b.total = flags_.addr() + 8;
   b.segShift = shift;
-  b.bigBegin = bigBegin;
-  b.entryBytes = entryBytes;
+  b.bigBegin = bigBegin_;
+  const uint32_t fillLds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
 
   hipCheck(api_, api_->EventRecord(evStart_, stream_), "hipEventRecord");
   size_t nev = 0;
   std::vector<bool> fillPhase;
-  for (size_t si = 0; si < slabs.size(); si++)
+  for (size_t si = 0; si < slabs_.size(); si++)
   {
-    const Slab& sl = slabs[si];
+    const Slab& sl = slabs_[si];
+    const uint64_t slabOff = offBase + offPos[si] * 4;
+    const uint64_t slabCount = listCount_.addr() + (uint64_t) sl.segBase * 4;
     if (sl.win1 > sl.win0)
     {
       b.windows = descriptors_.addr() + runBytes + sl.win0 * sizeof(PsgWindow);
       b.nWindows = (uint32_t) (sl.win1 - sl.win0);
       b.nWGs = sl.nWGs;
-      b.nLists = 8u * sl.nSegs;
-      b.capacity = sl.capacity;
-      b.maxWindowLists = sl.maxLists;
+      b.listOff = slabOff;
+      b.listCount = slabCount;
       void* pb[] = { &b };
       recordPhase(nev++);
       fillPhase.push_back(true);
-      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketCount_, sl.nWGs, 1, 1, 512, 1, 1,
-                                              sl.maxLists * 4, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_count)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnScanChunks_, (b.nLists + 3) / 4, 1, 1, 256, 1, 1,
-                                              0, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_scan_chunks)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnScanLists_, 1, 1, 1, 1024, 1, 1, 0, stream_, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_scan_lists)");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, sl.nWGs, 1, 1, 512, 1, 1,
-                                              sl.maxLists * 4, stream_, pb, nullptr),
+      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, sl.nWGs, 1, 1, PSG_BUCKET_THREADS, 1, 1,
+                                              fillLds, stream_, pb, nullptr),
                "hipModuleLaunchKernel(psg_bucket_fill)");
       args.buckets = entries_.addr();
-      args.capacity = sl.capacity;
     }
     else
-    {
       args.buckets = 0;
-      args.capacity = 0;
-    }
+    args.listOff = slabOff;
+    args.listCount = slabCount;
     args.runs = descriptors_.addr() + sl.run0 * sizeof(PsgRun);
     args.nRuns = (uint32_t) (sl.run1 - sl.run0);
     args.nSegs = sl.nSegs;
@@ -421,13 +451,11 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
   recordPhase(nev++);
   hipCheck(api_, api_->EventRecord(evStop_, stream_), "hipEventRecord");
   hostCounts_.resize((size_t) outs * PSG_NCOUNTS);
-  hipCheck(api_, api_->MemcpyAsync(hostOverflow_, overflow_.get(), 4, hipMemcpyDeviceToHost, stream_),
-           "hipMemcpyAsync overflow");
+  hipCheck(api_, api_->MemcpyAsync(hostFlags_, flags_.get(), 16, hipMemcpyDeviceToHost, stream_),
+           "hipMemcpyAsync flags");
   hipCheck(api_, api_->MemcpyAsync(hostCounts_.data(), counts_.get(), hostCounts_.size() * 8,
                                    hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync counts");
   hipCheck(api_, api_->StreamSynchronize(stream_), "sieve kernels");
-  if (*hostOverflow_)
-    throw primesieve_error("big-prime bucket capacity exceeded");
 
   float ms = 0.f;
   hipCheck(api_, api_->EventElapsedTime(&ms, evStart_, evStop_), "hipEventElapsedTime");
@@ -439,15 +467,11 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
     if (fillPhase[e]) fillMs += t;
     else sieveMs += t;
   }
-  std::copy(hostCounts_.begin(), hostCounts_.end(), out6);
-  stats_.kernel_ms = sieveMs;
-  stats_.fill_ms = fillMs;
-  stats_.device_ms = ms;
-  stats_.segments = segsTotal;
-  stats_.launches = (uint32_t) slabs.size();
-  stats_.sieving_primes = primesUpTo(isqrt64(maxHi));
-  stats_.wall_ms = std::chrono::duration<double, std::milli>(
-                       std::chrono::steady_clock::now() - t0).count();
+  stats_.kernel_ms += sieveMs;
+  stats_.fill_ms += fillMs;
+  stats_.device_ms += ms;
+  stats_.bucket_entries = hostFlags_[1];
+  return (hostFlags_[0] & 0xffffffffull) == 0;
 }
 
 void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
@@ -463,9 +487,9 @@ void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint
 {
   ExecOptions opt;
   opt.runSegs = std::min<uint32_t>(2048, std::max<uint32_t>(slabSegments_, 1));
-  opt.windowSegs = opt.runSegs;          // one prime sweep per run
-  opt.primesPerChunk = 16384;
-  opt.chunkListBudget = 1u << 19;        // <= 2 MiB of counts per window
+  opt.windowSegs = PSG_MAX_WINDOW_SEGS;  // few prime sweeps per run
+  opt.primesPerChunk = 4096;
+  opt.maxChunks = 16384;
   opt.slabSegs = std::max<uint32_t>(opt.runSegs, batchSlabSegs_);
   execute(iv, outs, out6, opt);
 }

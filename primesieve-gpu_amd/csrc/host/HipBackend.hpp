@@ -62,9 +62,8 @@ struct ExecOptions
   uint32_t runSegs = 1024;        // longest run
   uint32_t slabSegs = 1024;       // segments per slab (one persistent launch)
   uint32_t windowSegs = 64;       // segments per bucket window
-  uint32_t primesPerChunk = 2048; // big primes per count / fill workgroup ...
-  uint32_t maxChunks = 64;        // ... but at most max(maxChunks,
-  uint32_t chunkListBudget = 0;   //     chunkListBudget / lists) workgroups per window
+  uint32_t primesPerChunk = 512;  // big primes per fill workgroup ...
+  uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
   double entryBudget = 5.0e8;     // expected bucket entries per slab
 };
 
@@ -99,6 +98,7 @@ public:
     t.slab_segments = slabSegments_;
     t.bucket_factor = bucketFactor_ ? bucketFactor_ : 0xffffffffu;
     t.presieve = config_.preSieve ? 1u : 0u;
+    t.bucket_margin = bucketMargin_;
     return t;
   }
 
@@ -107,6 +107,7 @@ private:
   void ensureSievingPrimes(uint32_t limit);
   void execute(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6,
                const ExecOptions& opt);
+  bool launchPlan(uint32_t outs, const std::vector<uint32_t>& room);
   uint32_t primesUpTo(uint64_t x) const;
   uint32_t persistentGrid(uint32_t nSegs);
   void recordPhase(size_t i);
@@ -122,12 +123,10 @@ private:
   uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
   uint32_t windowSegs_ = 64;      // segments per bucket window of a single interval
   uint32_t persistentWGs_ = 0;    // resident workgroups of the sieve kernel
+  uint32_t bucketMargin_ = 6;     // bucket list room: expected + margin * sqrt(expected) + 64
 
   hipModule_t module_ = nullptr;
   hipFunction_t fnSieve_ = nullptr;
-  hipFunction_t fnBucketCount_ = nullptr;
-  hipFunction_t fnScanChunks_ = nullptr;
-  hipFunction_t fnScanLists_ = nullptr;
   hipFunction_t fnBucketFill_ = nullptr;
   hipStream_t stream_ = nullptr;
   hipEvent_t evStart_ = nullptr;
@@ -140,14 +139,26 @@ private:
   DeviceBuffer counts_;        // uint64 [outputs][6]
   DeviceBuffer descriptors_;   // runs and windows of every slab of a call
   DeviceBuffer workCounters_;  // uint32 per slab
-  DeviceBuffer bucketCounts_;  // uint32 counts blocks
-  DeviceBuffer listOff_;       // uint32 [lists + 1]
-  DeviceBuffer entries_;       // bucket entries
-  DeviceBuffer overflow_;      // uint32 flag
-  uint32_t* hostOverflow_ = nullptr;   // pinned
+  DeviceBuffer listCount_;     // uint32 per segment of the call: bucket entries appended
+  DeviceBuffer entries_;       // bucket entries of one slab
+  DeviceBuffer flags_;         // uint32 overflow flag, pad, uint64 entries appended
+  uint64_t* hostFlags_ = nullptr;      // pinned copy of flags_
   std::vector<uint64_t> hostCounts_;
   std::vector<unsigned char> staging_;
   std::vector<hipEvent_t> phaseEvents_;
+
+  // the plan of the current call (built by execute, launched by launchPlan)
+  struct Slab
+  {
+    size_t run0, run1, win0, win1;
+    uint32_t nSegs, nWGs, segBase;
+    double est;
+  };
+  std::vector<PsgRun> runs_;
+  std::vector<PsgWindow> wins_;
+  std::vector<Slab> slabs_;
+  std::vector<double> segExpect_;    // expected bucket entries per segment (0: none)
+  uint32_t bigBegin_ = 0;
 
   std::vector<uint32_t> hostPrimes_; // the primes held in primes_
   std::vector<double> invPrefix_;    // invPrefix_[i] = sum_{j<i} 1/p_j

@@ -90,6 +90,7 @@ int psg_set_gpu_tuning(const psg_tuning* t)
   c.segmentsPerLaunch = t->segments_per_launch;
   c.slabSegments = t->slab_segments;
   c.bucketFactor = t->bucket_factor;
+  c.bucketMargin = t->bucket_margin;
   c.preSieve = t->presieve != 0;
   primesieve::gpu::setConfig(c);
   return 0;

@@ -48,8 +48,9 @@ typedef struct PsgSieveArgs {
   uint64_t tile0;       /* const uint32_t*: sliding windows (0 = no pre-sieve) */
   uint64_t tile1;       /* const uint32_t*: sliding windows */
   uint64_t counts;      /* uint64_t[outs][PSG_NCOUNTS], accumulated atomically */
-  uint64_t buckets;     /* bucket entries of the slab (0 = none) */
-  uint64_t listOff;     /* const uint32_t*[8 * nSegs + 1] */
+  uint64_t buckets;     /* const uint32_t*: bucket entries of the slab (0 = none) */
+  uint64_t listOff;     /* const uint32_t[nSegs + 1]: segment s owns entries [listOff[s], listOff[s+1]) */
+  uint64_t listCount;   /* const uint32_t[nSegs]: entries appended to segment s (may exceed its room) */
   uint64_t workCounter; /* uint32_t*, zero at launch */
   uint32_t nRuns;
   uint32_t nSegs;       /* segments in the slab */
@@ -59,48 +60,49 @@ typedef struct PsgSieveArgs {
   uint32_t tier2End;    /* primes[tier1End, tier2End): p <= segBytes */
   uint32_t tier3End;    /* primes[tier2End, tier3End): per-segment big primes;
                            larger primes come from the bucket lists */
-  uint32_t entryBytes;  /* 2: uint16 byte offsets (segBytes <= 65536); 4: uint32 */
-  uint32_t capacity;    /* bucket entries allocated */
   uint32_t pad;
 } PsgSieveArgs;
+
+/* Big-prime bucket tier.  A bucket entry is the bit index of one crossing
+ * inside its segment, byte * 8 + bit (word = entry >> 5, bit = entry & 31),
+ * appended to the list of that segment.  Lists have a fixed room each
+ * (listOff, sized from the expected number of crossings plus a margin) and
+ * a counter (listCount) that psg_bucket_fill advances atomically; a counter
+ * that ends above its room raises *overflow and the host repeats the call
+ * with rooms equal to the counted totals. */
+#define PSG_BUCKET_THREADS 256u      /* threads of a psg_bucket_fill workgroup */
+#define PSG_BUCKET_HITS 16u          /* hits a fill thread stages per round */
+#define PSG_BUCKET_STAGE (PSG_BUCKET_THREADS * PSG_BUCKET_HITS)
+#define PSG_MAX_WINDOW_SEGS PSG_BUCKET_THREADS   /* segments per bucket window */
+#define PSG_ENTRY_SEG_SHIFT 21u      /* staged entry = window segment << 21 | bit index */
 
 /* One bucket window (32 bytes): consecutive segments of one run walked by
  * nChunks workgroups, each over an interleaved share of the big primes
  * primes[bigBegin, bigEnd). */
 typedef struct PsgWindow {
   uint64_t base;        /* global byte of the window's first byte */
-  uint32_t firstList;   /* slab list index of its list 0 (8 * slab segment) */
-  uint32_t segs;        /* segments in the window */
+  uint32_t firstSeg;    /* slab segment of the window's first segment */
+  uint32_t segs;        /* segments in the window, <= PSG_MAX_WINDOW_SEGS */
   uint32_t bigEnd;      /* exclusive prime index bound of its interval */
   uint32_t nChunks;     /* workgroups walking this window */
-  uint32_t countsBase;  /* offset of its counts block [8 * segs][nChunks] (list-major) */
-  uint32_t wgBase;      /* first workgroup of the window in the count / fill grid */
+  uint32_t wgBase;      /* first workgroup of the window in the fill grid */
+  uint32_t pad;
 } PsgWindow;
 
-/* Big-prime bucket tier of one slab.  Hits go to list (segment * 8 + class),
- * class = bit index.  Lists are laid out by a counting sort with no global
- * atomics:
- *   psg_bucket_count        per-workgroup LDS histogram -> counts
- *   psg_bucket_scan_chunks  exclusive prefix over a window's chunks per list
- *   psg_bucket_scan_lists   exclusive prefix over all lists -> listOff
- *   psg_bucket_fill         re-walk with LDS cursors, store byte offsets
- * psg_sieve_count then strikes list (s*8 + j) of segment s with bit j. */
+/* Arguments of psg_bucket_fill (one workgroup per (window, chunk)). */
 typedef struct PsgBucketArgs {
-  uint64_t windows;     /* const PsgWindow*, sorted by wgBase (and firstList) */
+  uint64_t windows;     /* const PsgWindow*, sorted by wgBase */
   uint64_t primes;      /* const uint32_t* */
   uint64_t magics;      /* const uint64_t* */
-  uint64_t counts;      /* uint32_t counts blocks */
-  uint64_t listOff;     /* uint32_t[nLists + 1] */
-  uint64_t entries;     /* uint16_t* or uint32_t* (entryBytes) */
-  uint64_t overflow;    /* uint32_t flag, set when capacity is exceeded */
+  uint64_t listOff;     /* const uint32_t[nSegs + 1] */
+  uint64_t listCount;   /* uint32_t[nSegs], zero at launch */
+  uint64_t entries;     /* uint32_t* */
+  uint64_t overflow;    /* uint32_t flag, set when a list outgrows its room */
+  uint64_t total;       /* unsigned long long: entries appended (statistics) */
   uint32_t nWindows;
-  uint32_t nWGs;        /* workgroups of the count / fill grid */
-  uint32_t nLists;      /* 8 * segments in the slab */
+  uint32_t nWGs;        /* workgroups of the fill grid */
   uint32_t segShift;    /* log2(segBytes) */
   uint32_t bigBegin;
-  uint32_t capacity;    /* entries allocated */
-  uint32_t entryBytes;  /* 2 or 4 */
-  uint32_t maxWindowLists; /* LDS counters per workgroup */
 } PsgBucketArgs;
 
 #endif

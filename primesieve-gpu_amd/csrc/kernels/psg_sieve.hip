@@ -75,6 +75,12 @@ __device__ __forceinline__ void strike(uint32_t* sieve, uint32_t byteIdx, uint32
   atomicAnd(&sieve[byteIdx >> 2], ~(1u << (((byteIdx & 3u) << 3) + bit)));
 }
 
+// Clear bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
+__device__ __forceinline__ void strike_bit(uint32_t* sieve, uint32_t v)
+{
+  atomicAnd(&sieve[v >> 5], ~(1u << (v & 31u)));
+}
+
 __device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
 {
   return (x << r) | (x >> (32u - r));
@@ -345,34 +351,23 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     }
   }
 
-  // ---- 2b. big-prime bucket lists of this segment -----------------------
-  // Four independent loads per thread are in flight before their strikes.
+  // ---- 2b. big-prime bucket list of this segment ------------------------
+  // Entries are bit indices (byte * 8 + bit); four independent loads per
+  // thread are in flight before their strikes.
   if (args.buckets) {
     const uint32_t* __restrict__ offs = (const uint32_t*) args.listOff;
-    for (uint32_t c = 0; c < 8; c++) {
-      const uint32_t e0 = min(offs[seg * 8u + c], args.capacity);
-      const uint32_t e1 = min(offs[seg * 8u + c + 1u], args.capacity);
-      uint32_t e = e0 + tid;
-      if (args.entryBytes == 2) {
-        const uint16_t* __restrict__ ent = (const uint16_t*) args.buckets;
-        for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
-          const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
-          const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
-          strike(sieve, v0, c); strike(sieve, v1, c);
-          strike(sieve, v2, c); strike(sieve, v3, c);
-        }
-        for (; e < e1; e += nthreads) strike(sieve, ent[e], c);
-      } else {
-        const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
-        for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
-          const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
-          const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
-          strike(sieve, v0, c); strike(sieve, v1, c);
-          strike(sieve, v2, c); strike(sieve, v3, c);
-        }
-        for (; e < e1; e += nthreads) strike(sieve, ent[e], c);
-      }
+    const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
+    const uint32_t e0 = offs[seg];
+    const uint32_t room = offs[seg + 1u] - e0;
+    const uint32_t e1 = e0 + min(((const uint32_t*) args.listCount)[seg], room);
+    uint32_t e = e0 + tid;
+    for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
+      const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
+      const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
+      strike_bit(sieve, v0); strike_bit(sieve, v1);
+      strike_bit(sieve, v2); strike_bit(sieve, v3);
     }
+    for (; e < e1; e += nthreads) strike_bit(sieve, ent[e]);
   }
   __syncthreads();
 
@@ -435,56 +430,91 @@ __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
   return (uint32_t) ((r < 20u ? (kBitLo >> (3u * r)) : (kBitHi >> (3u * (r - 20u)))) & 7u);
 }
 
-// Walk every hit of window w's chunk-c primes over the window; call
-// emit(windowList, byteOffsetInSegment) per hit.  The big primes are cut into
-// tiles of blockDim consecutive primes and the tiles are dealt round-robin to
-// the chunks (tile m*nChunks + c), so that every wave loads consecutive primes
-// (coalesced) and every chunk gets a similar mix of small and large primes.
-//   dense primes (>= 8 multiples in the window): the 8 residue classes, each
-//     an arithmetic progression of bytes with stride p;
-//   sparse primes: the multiples of p in the window's value range directly,
-//     keeping those whose cofactor is coprime to 30 -- one small division
-//     instead of eight class starts.
+// Resumable walk of one thread over its share of a window's big primes.  The
+// big primes are cut into tiles of blockDim consecutive primes and the tiles
+// are dealt round-robin to the window's chunks (tile m*nChunks + c), so that
+// every wave loads consecutive primes (coalesced) and every chunk gets a
+// similar mix of small and large primes.  Per prime:
+//   dense (>= 8 multiples in the window): the 8 residue classes, each an
+//     arithmetic progression of bytes with stride p;
+//   sparse: the multiples of p in the window's value range directly, keeping
+//     those whose cofactor is coprime to 30 -- one small division instead of
+//     eight class starts.
+// A round stops after `limit` hits; the walk resumes there in the next round.
+struct BucketWalk {
+  uint32_t k;        // index of the prefetched prime
+  uint32_t pn;       // the prefetched prime primes[k] (0: none left)
+  uint64_t Mn;       // its division magic
+  uint32_t p;        // prime being walked (0: none)
+  uint32_t R;        // window base mod p
+  uint32_t j;        // dense: current class; 8 marks a sparse prime
+  uint32_t pos;      // dense: next byte of class j, relative to the window
+  uint32_t qm;       // sparse: cofactor of the next multiple, mod 30
+  uint64_t off;      // sparse: next multiple minus the window's first value
+};
+
+// emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ...; returns the
+// number of hits of this round.
 template <typename Emit>
-__device__ __forceinline__ void bucket_walk(const PsgBucketArgs& a, const PsgWindow& w, uint32_t c,
-                                            const uint16_t* ctab, Emit emit)
+__device__ __forceinline__ uint32_t bucket_walk_round(const PsgBucketArgs& a, const PsgWindow& w,
+                                                      BucketWalk& s, uint32_t stride,
+                                                      const uint16_t* ctab, uint32_t limit,
+                                                      Emit emit)
 {
   const uint32_t* __restrict__ primes = (const uint32_t*) a.primes;
   const uint64_t* __restrict__ magics = (const uint64_t*) a.magics;
   const uint32_t segMask = (1u << a.segShift) - 1u;
-  const uint32_t stride = blockDim.x * w.nChunks;
   const uint32_t bytes = w.segs << a.segShift;
   const uint64_t V = 30ull * bytes;               // values covered by the window
   const uint64_t v0 = 30ull * w.base + 7u;        // its first value
-  for (uint32_t k = a.bigBegin + c * blockDim.x + threadIdx.x; k < w.bigEnd; k += stride) {
-    const uint32_t p = primes[k];
-    const uint32_t R = mod_magic(w.base, p, magics[k]);
-    if ((uint64_t) p * 8u > V) {
-      // v0 = 30*Q*p + x with x = 30R + 7 < 31p; the first multiple of p at or
-      // after v0 is v0 + off with cofactor 30*Q + ceil(x / p)
-      const uint64_t x = 30ull * R + 7u;
-      uint32_t t = (uint32_t) ((float) x / (float) p);
-      while ((uint64_t) t * p > x) t--;
-      while ((uint64_t) (t + 1u) * p <= x) t++;
-      const uint32_t u = (uint32_t) (x - (uint64_t) t * p);
-      uint64_t off = u ? (uint64_t) (p - u) : 0ull;
-      uint32_t qm = (t + (u ? 1u : 0u)) % 30u;
-      for (; off < V; off += p) {
-        if (((kCoprime30 >> qm) & 1u) && v0 + off != (uint64_t) p) {
-          const uint32_t byte = (uint32_t) (off / 30u);
-          const uint32_t bit = bit_of_residue((uint32_t) (off - 30ull * byte));
-          emit(((byte >> a.segShift) << 3) + bit, byte & segMask);
-        }
-        qm = (qm == 29u) ? 0u : qm + 1u;
+  uint32_t n = 0;
+  while (n < limit) {
+    if (s.p == 0u) {
+      if (s.pn == 0u) break;
+      // take the prefetched prime and fetch the next one while this one is walked
+      const uint32_t p = s.pn;
+      const uint32_t R = mod_magic(w.base, p, s.Mn);
+      s.k += stride;
+      s.pn = (s.k < w.bigEnd) ? primes[s.k] : 0u;
+      s.Mn = (s.k < w.bigEnd) ? magics[s.k] : 0ull;
+      s.p = p;
+      s.R = R;
+      if ((uint64_t) p * 8u > V) {
+        // v0 = 30*Q*p + x with x = 30R + 7 < 31p; the first multiple of p at
+        // or after v0 is v0 + off with cofactor 30*Q + ceil(x / p)
+        const uint64_t x = 30ull * R + 7u;
+        uint32_t t = (uint32_t) ((float) x / (float) p);
+        while ((uint64_t) t * p > x) t--;
+        while ((uint64_t) (t + 1u) * p <= x) t++;
+        const uint32_t u = (uint32_t) (x - (uint64_t) t * p);
+        s.off = u ? (uint64_t) (p - u) : 0ull;
+        s.qm = (t + (u ? 1u : 0u)) % 30u;
+        s.j = 8u;
+      } else {
+        s.j = 0u;
+        s.pos = class_start(p, R, w.base, 0u, ctab);
       }
+    }
+    if (s.j == 8u) {
+      for (; n < limit && s.off < V; s.off += s.p) {
+        if (((kCoprime30 >> s.qm) & 1u) && v0 + s.off != (uint64_t) s.p) {
+          const uint32_t byte = (uint32_t) (s.off / 30u);
+          const uint32_t bit = bit_of_residue((uint32_t) (s.off - 30ull * byte));
+          emit(n++, byte >> a.segShift, ((byte & segMask) << 3) | bit);
+        }
+        s.qm = (s.qm == 29u) ? 0u : s.qm + 1u;
+      }
+      if (s.off >= V) s.p = 0u;
     } else {
-      for (uint32_t j = 0; j < 8; j++) {
-        const uint32_t i = class_start(p, R, w.base, j, ctab);
-        for (uint32_t pos = i; pos < bytes; pos += p)
-          emit(((pos >> a.segShift) << 3) + j, pos & segMask);
+      for (; n < limit && s.pos < bytes; s.pos += s.p)
+        emit(n++, s.pos >> a.segShift, ((s.pos & segMask) << 3) | s.j);
+      if (s.pos >= bytes) {
+        if (++s.j == 8u) s.p = 0u;
+        else s.pos = class_start(s.p, s.R, w.base, s.j, ctab);
       }
     }
   }
+  return n;
 }
 
 __device__ __forceinline__ const PsgWindow& window_of_wg(const PsgBucketArgs& a, uint32_t& chunk)
@@ -529,115 +559,106 @@ psg_sieve_count(PsgSieveArgs args)
 // Big-prime bucket tier (see PsgBucketArgs)
 // ---------------------------------------------------------------------------
 
-extern "C" __global__ void __launch_bounds__(512)
-psg_bucket_count(PsgBucketArgs a)
-{
-  extern __shared__ uint32_t hist[];
-  __shared__ uint16_t ctab[240];
-  init_ctab(ctab);
-  uint32_t c;
-  const PsgWindow w = window_of_wg(a, c);
-  const uint32_t nl = w.segs * 8u;
-  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) hist[l] = 0;
-  __syncthreads();
-  bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t) { atomicAdd(&hist[list], 1u); });
-  __syncthreads();
-  // list-major counts block: counts[l * nChunks + c]
-  uint32_t* counts = (uint32_t*) a.counts + w.countsBase + c;
-  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) counts[(uint64_t) l * w.nChunks] = hist[l];
-}
+constexpr uint32_t kStageF = PSG_BUCKET_HITS;   // hits per thread per round
 
-// One wave per slab list l: counts[l][c] <- sum_{c' < c} counts[l][c'] over
-// its window's chunks (a contiguous row), listOff[l] <- the row total (0 for
-// lists of runs without big primes).  Rows are scanned 64 chunks at a time
-// with a shuffle prefix sum.
-extern "C" __global__ void __launch_bounds__(256)
-psg_bucket_scan_chunks(PsgBucketArgs a)
-{
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t l = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (l >= a.nLists) return;
-  const PsgWindow* win = (const PsgWindow*) a.windows;
-  uint32_t carry = 0;
-  if (a.nWindows && win[0].firstList <= l) {
-    const PsgWindow& w = win[last_le(a.nWindows, l, [&](uint32_t i) { return win[i].firstList; })];
-    if (l < w.firstList + w.segs * 8u) {
-      uint32_t* row = (uint32_t*) a.counts + w.countsBase + (uint64_t) (l - w.firstList) * w.nChunks;
-      for (uint32_t c0 = 0; c0 < w.nChunks; c0 += 64u) {
-        const uint32_t c = c0 + lane;
-        const uint32_t v = (c < w.nChunks) ? row[c] : 0u;
-        uint32_t x = v;
-        for (uint32_t d = 1; d < 64u; d <<= 1) {
-          const uint32_t y = (uint32_t) __shfl_up((int) x, d);
-          if (lane >= d) x += y;
-        }
-        if (c < w.nChunks) row[c] = carry + x - v;
-        carry += (uint32_t) __builtin_amdgcn_readlane((int) x, 63);
-      }
-    }
-  }
-  if (lane == 0) ((uint32_t*) a.listOff)[l] = carry;
-}
-
-// listOff[0..nLists] <- exclusive prefix sum of the list totals (one block)
-extern "C" __global__ void __launch_bounds__(1024)
-psg_bucket_scan_lists(PsgBucketArgs a)
-{
-  __shared__ uint32_t part[1024];
-  uint32_t* off = (uint32_t*) a.listOff;
-  const uint32_t n = a.nLists;
-  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-  const uint32_t b = min(threadIdx.x * per, n);
-  const uint32_t e = min(b + per, n);
-  uint32_t sum = 0;
-  for (uint32_t i = b; i < e; i++) sum += off[i];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  // Hillis-Steele inclusive scan of the partial sums
-  for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
-    const uint32_t v = (threadIdx.x >= d) ? part[threadIdx.x - d] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[threadIdx.x] - sum;
-  for (uint32_t i = b; i < e; i++) {
-    const uint32_t v = off[i];
-    off[i] = run;
-    run += v;
-  }
-  if (threadIdx.x == blockDim.x - 1) {
-    off[n] = part[threadIdx.x];
-    if (part[threadIdx.x] > a.capacity) *(uint32_t*) a.overflow = 1u;
-  }
-}
-
-extern "C" __global__ void __launch_bounds__(512)
+// One pass per (window, chunk) workgroup, in rounds.  In a round every thread
+// walks up to kStageF hits into its LDS stage slots while an LDS histogram
+// counts them per window segment.  Then each segment's run of the round takes
+// its place in the segment's list with one global atomic (issued early, used
+// late), the hits are sorted by segment in place (every thread first holds
+// its hits in registers), and the sorted runs are written out contiguously:
+// coalesced stores instead of one scattered store per hit.
+extern "C" __global__ void __launch_bounds__(PSG_BUCKET_THREADS)
 psg_bucket_fill(PsgBucketArgs a)
 {
-  extern __shared__ uint32_t cursor[];
+  extern __shared__ uint32_t lds[];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthreads = PSG_BUCKET_THREADS;        // the launch's block size
+  uint32_t* stage = lds;                               // [kStageF][nthreads], then sorted
+  uint32_t* hist = stage + PSG_BUCKET_STAGE;           // [W] counts, then scatter cursors
+  uint32_t* place = hist + PSG_MAX_WINDOW_SEGS;        // [W] first sorted slot | fit << 16
+  uint32_t* dst = place + PSG_MAX_WINDOW_SEGS;         // [W] entry index of the segment's run
   __shared__ uint16_t ctab[240];
+  __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
+
   init_ctab(ctab);
   uint32_t c;
   const PsgWindow w = window_of_wg(a, c);
-  const uint32_t nl = w.segs * 8u;
-  const uint32_t* counts = (const uint32_t*) a.counts + w.countsBase + c;
-  const uint32_t* listOff = (const uint32_t*) a.listOff + w.firstList;
-  for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x)
-    cursor[l] = listOff[l] + counts[(uint64_t) l * w.nChunks];
+  const uint32_t W = w.segs;                           // <= PSG_MAX_WINDOW_SEGS <= blockDim
+  const uint32_t* __restrict__ listOff = (const uint32_t*) a.listOff + w.firstSeg;
+  uint32_t* listCount = (uint32_t*) a.listCount + w.firstSeg;
+  uint32_t* entries = (uint32_t*) a.entries;
+  const uint32_t lane = tid & 63u, wave = tid >> 6, nwaves = nthreads >> 6;
+  for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
+
+  BucketWalk st;
+  st.k = a.bigBegin + c * nthreads + tid;
+  st.pn = (st.k < w.bigEnd) ? ((const uint32_t*) a.primes)[st.k] : 0u;
+  st.Mn = (st.k < w.bigEnd) ? ((const uint64_t*) a.magics)[st.k] : 0ull;
+  st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0ull;
+  const uint32_t stride = nthreads * w.nChunks;
+  uint64_t appended = 0;
   __syncthreads();
-  const uint32_t cap = a.capacity;
-  if (a.entryBytes == 2) {
-    uint16_t* ent = (uint16_t*) a.entries;
-    bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t off) {
-      const uint32_t slot = atomicAdd(&cursor[list], 1u);
-      if (slot < cap) ent[slot] = (uint16_t) off;
-    });
-  } else {
-    uint32_t* ent = (uint32_t*) a.entries;
-    bucket_walk(a, w, c, ctab, [&](uint32_t list, uint32_t off) {
-      const uint32_t slot = atomicAdd(&cursor[list], 1u);
-      if (slot < cap) ent[slot] = off;
-    });
+  for (;;) {
+    const uint32_t n = bucket_walk_round(a, w, st, stride, ctab, kStageF,
+        [&](uint32_t i, uint32_t seg, uint32_t bitIdx) {
+          stage[i * nthreads + tid] = (seg << PSG_ENTRY_SEG_SHIFT) | bitIdx;
+          atomicAdd(&hist[seg], 1u);
+        });
+    const int more = __syncthreads_or((st.p != 0u || st.pn != 0u) ? 1 : 0);
+
+    // exclusive prefix of the counts; thread i owns segment i
+    const uint32_t cnt = (tid < W) ? hist[tid] : 0u;
+    uint32_t x = cnt;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+      const uint32_t y = (uint32_t) __shfl_up((int) x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63u) waveTotal[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t v = 0; v < nwaves; v++) {
+      const uint32_t t = waveTotal[v];
+      before += (v < wave) ? t : 0u;
+      total += t;
+    }
+    const uint32_t f = before + x - cnt;
+    uint32_t at = 0;
+    if (tid < W) {
+      hist[tid] = f;
+      if (cnt) at = atomicAdd(&listCount[tid], cnt);   // consumed after the scatter
+    }
+    uint32_t e[kStageF];
+#pragma unroll
+    for (uint32_t u = 0; u < kStageF; u++) e[u] = stage[u * nthreads + tid];
+    __syncthreads();                                   // cursors set, stage read
+
+#pragma unroll
+    for (uint32_t u = 0; u < kStageF; u++)
+      if (u < n) stage[atomicAdd(&hist[e[u] >> PSG_ENTRY_SEG_SHIFT], 1u)] = e[u];
+    if (tid < W) {
+      const uint32_t room = listOff[tid + 1u] - listOff[tid];
+      if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
+      const uint32_t fit = (at >= room) ? 0u : min(cnt, room - at);
+      place[tid] = f | (fit << 16);
+      dst[tid] = listOff[tid] + at;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (uint32_t u = 0; u < kStageF; u++) {
+      const uint32_t i = u * nthreads + tid;
+      const uint32_t v = stage[i];
+      const uint32_t seg = (v >> PSG_ENTRY_SEG_SHIFT) & (PSG_MAX_WINDOW_SEGS - 1u);
+      const uint32_t pl = place[seg];
+      const uint32_t r = i - (pl & 0xffffu);
+      if (i < total && r < (pl >> 16)) entries[dst[seg] + r] = v & ((1u << PSG_ENTRY_SEG_SHIFT) - 1u);
+    }
+    appended += total;
+    for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
+    __syncthreads();
+    if (!more) break;
   }
+  if (tid == 0 && appended)
+    atomicAdd((unsigned long long*) a.total, (unsigned long long) appended);
 }

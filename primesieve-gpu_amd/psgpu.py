@@ -37,6 +37,8 @@ class Stats(ctypes.Structure):
         ("sieve_bytes", ctypes.c_uint32),
         ("work_group_size", ctypes.c_uint32),
         ("sieving_primes", ctypes.c_uint32),
+        ("bucket_retries", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -82,6 +84,7 @@ class Tuning(ctypes.Structure):
         ("slab_segments", ctypes.c_uint32),
         ("bucket_factor", ctypes.c_uint32),
         ("presieve", ctypes.c_uint32),
+        ("bucket_margin", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -195,10 +198,10 @@ def set_presieve(enable):
 
 
 def set_tuning(sieve_bytes=0, work_group_size=0, segments_per_launch=0, slab_segments=0,
-               bucket_factor=0, presieve=True):
+               bucket_factor=0, presieve=True, bucket_margin=0):
     """Set the whole tuning vector (0 = auto); recreates the backend."""
     t = Tuning(sieve_bytes, work_group_size, segments_per_launch, slab_segments,
-               bucket_factor, 1 if presieve else 0)
+               bucket_factor, 1 if presieve else 0, bucket_margin)
     if lib().psg_set_gpu_tuning(ctypes.byref(t)) != 0:
         raise PrimesieveError("psg_set_gpu_tuning failed")
 

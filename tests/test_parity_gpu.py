@@ -142,3 +142,26 @@ def test_try_count_policy(gpu):
         assert gpu.try_count(0, 10 ** 9, 1) == 3424506
     finally:
         gpu.set_enabled(False)
+
+
+def test_bucket_list_room_retry(gpu, oracle):
+    """Bucket lists get a room of E + m*sqrt(E) + 64 entries (E = expected
+    crossings of the segment).  With m = 1 many lists of a high interval
+    outgrow their rooms: the call must repeat once with the counted rooms and
+    still give the oracle's counts."""
+    a, b = 10 ** 13, 10 ** 13 + 10 ** 9
+    want = oracle.count6(a, b)
+    assert gpu.count_all(a, b) == want
+    st = gpu.last_stats()
+    assert st.bucket_retries == 0 and st.bucket_entries > 0
+    entries = st.bucket_entries
+    gpu.set_tuning(bucket_margin=1)
+    try:
+        assert gpu.get_tuning()["bucket_margin"] == 1
+        assert gpu.count_all(a, b) == want
+        st = gpu.last_stats()
+        _say("margin 1:", st.as_dict())
+        assert st.bucket_retries == 1
+        assert st.bucket_entries == entries
+    finally:
+        gpu.set_tuning()
