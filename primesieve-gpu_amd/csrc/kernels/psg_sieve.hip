@@ -38,6 +38,15 @@
 
 namespace {
 
+// Device addresses travel as uint64_t in the argument blocks; these views
+// make their loads and stores global (not flat) memory instructions, which
+// keeps them off the LDS wait counter.
+#define PSG_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const PSG_GLOBAL T* gptr(uint64_t a) { return (const PSG_GLOBAL T*) (uintptr_t) a; }
+template <typename T>
+__device__ __forceinline__ PSG_GLOBAL T* gptr_rw(uint64_t a) { return (PSG_GLOBAL T*) (uintptr_t) a; }
+
 __device__ __constant__ uint32_t kOffsets[8] = {7, 11, 13, 17, 19, 23, 29, 31};
 
 // R = B mod p for B < 2^63 using M = ceil(2^(64+L)/p):
@@ -53,7 +62,9 @@ __device__ __forceinline__ uint32_t mod_magic(uint64_t B, uint32_t p, uint64_t M
 // p, given R = B mod p.  ctab[(p % 30) * 8 + j] packs q_j (the cofactor
 // residue mod 30 of class j) in bits 0..7 and e_j + 1 in bits 8..15, where
 // c_j = (p/30)*q_j + e_j is the global byte of p*q_j.  The result is < p,
-// except that the prime itself (q_j == 1, first period) is skipped.
+// except that the prime itself (q_j == 1, first period) is skipped; that
+// can only happen when B < p, so callers that know B >= p pass kNear = false.
+template <bool kNear = true>
 __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t B,
                                                 uint32_t j, const uint16_t* ctab)
 {
@@ -63,16 +74,32 @@ __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t
   const uint32_t q = ent & 0xffu;
   const uint32_t c = a * q + (ent >> 8) - 1u;
   uint32_t i = (c >= R) ? c - R : c - R + p;
-  if (q == 1u && B + i < (uint64_t) p) {
+  if (kNear && q == 1u && B + i < (uint64_t) p) {
     const uint64_t ii = (uint64_t) i + p;
     i = (ii > 0xffffffffull) ? 0xffffffffu : (uint32_t) ii;
   }
   return i;
 }
 
-__device__ __forceinline__ void strike(uint32_t* sieve, uint32_t byteIdx, uint32_t bit)
+// The <= 8 hits of a prime S < p in the segment at B (R = B mod p): one per
+// residue class at most.  The class table row of p is one 16-byte LDS read.
+template <bool kNear>
+__device__ __forceinline__ void strike_big(uint32_t* sieve, uint32_t p, uint32_t R, uint64_t B,
+                                           uint32_t S, const uint16_t* ctab)
 {
-  atomicAnd(&sieve[byteIdx >> 2], ~(1u << (((byteIdx & 3u) << 3) + bit)));
+  const uint32_t a = p / 30u;
+  const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
+  const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
+#pragma unroll
+  for (uint32_t c = 0; c < 8; c++) {
+    const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
+    const uint32_t q = ent & 0xffu;
+    const uint32_t cc = a * q + (ent >> 8) - 1u;
+    uint32_t i = (cc >= R) ? cc - R : cc - R + p;
+    if (kNear && q == 1u && B + i < (uint64_t) p)
+      i = 0xffffffffu;                  // the prime itself; its next hit is beyond S
+    if (i < S) atomicAnd(&sieve[i >> 2], ~(1u << (((i & 3u) << 3) + c)));
+  }
 }
 
 // Clear bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
@@ -171,10 +198,10 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   const uint32_t words = S >> 2;
   const uint64_t B = run.B0 + (uint64_t) (seg - run.segStart) * S;
 
-  const uint32_t* __restrict__ primes = (const uint32_t*) args.primes;
-  const uint64_t* __restrict__ magics = (const uint64_t*) args.magics;
-  const uint32_t* __restrict__ tile0 = (const uint32_t*) args.tile0;
-  const uint32_t* __restrict__ tile1 = (const uint32_t*) args.tile1;
+  const auto* __restrict__ primes = gptr<uint32_t>(args.primes);
+  const auto* __restrict__ magics = gptr<uint64_t>(args.magics);
+  const auto* __restrict__ tile0 = gptr<uint32_t>(args.tile0);
+  const auto* __restrict__ tile1 = gptr<uint32_t>(args.tile1);
 
   // ---- 1. fill from the pre-sieve tiles ----------------------------------
   if (!tile0) {
@@ -229,6 +256,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     const uint32_t nwaves = nthreads >> 6;
     const uint32_t j = lane & 7u;
     const uint32_t slot = lane >> 3;
+    const bool nearSmall = B < (uint64_t) S;   // tiers 0-2 hold primes <= S
 
     // tier 0 (p <= 256): every wave strikes all of them over its own slice
     // of the segment, so the heaviest primes are shared evenly.
@@ -240,7 +268,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
       for (uint32_t b = 0; b < t0; b += 8) {
         const uint32_t k = b + slot;
         const uint32_t p = (k < t0) ? primes[k] : 0u;
-        const uint32_t st = p ? class_start(p, mod_magic(Bs, p, magics[k]), Bs, j, ctab) : 0u;
+        const uint32_t R = p ? mod_magic(Bs, p, magics[k]) : 0u;
+        const uint32_t st = !p ? 0u : nearSmall ? class_start<true>(p, R, Bs, j, ctab)
+                                               : class_start<false>(p, R, Bs, j, ctab);
         // all eight permutes first: one wait for the batch, not one per prime
         uint32_t sq[8];
 #pragma unroll
@@ -270,7 +300,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         kN = t0 + bn * 8u + slot;
         pN = (bn < nb && kN < t1) ? primes[kN] : 0u;
         MN = (bn < nb && kN < t1) ? magics[kN] : 0ull;
-        const uint32_t st = p ? class_start(p, mod_magic(B, p, M), B, j, ctab) : 0u;
+        const uint32_t R = p ? mod_magic(B, p, M) : 0u;
+        const uint32_t st = !p ? 0u : nearSmall ? class_start<true>(p, R, B, j, ctab)
+                                               : class_start<false>(p, R, B, j, ctab);
         uint32_t sq[8];
 #pragma unroll
         for (uint32_t q = 0; q < 8; q++) sq[q] = (uint32_t) __shfl((int) st, (int) (q * 8u + j));
@@ -299,7 +331,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         pN = (bn < nb && kN < t2) ? primes[kN] : 0u;
         MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
         if (p) {
-          uint32_t i = class_start(p, mod_magic(B, p, M), B, j, ctab);
+          const uint32_t R = mod_magic(B, p, M);
+          uint32_t i = nearSmall ? class_start<true>(p, R, B, j, ctab)
+                                 : class_start<false>(p, R, B, j, ctab);
           // two strikes per trip: the second mask is the first rotated by
           // 8 * (p mod 4); after two steps the rotation is always 16 bits
           uint32_t mask = ~(1u << (((i & 3u) << 3) + j));
@@ -315,11 +349,11 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     }
 
     // tier 3 (S < p <= bucket threshold): lane = prime, <= 1 hit per class.
-    // The class table row of p is one 16-byte LDS read; the prime itself can
-    // only be in range while B < 2^32 (uniform test).
+    // The prime itself can only be in the segment while B < p (uniform test
+    // against the largest tier-3 prime).
     {
       const uint32_t nb = (t3 - t2 + 63u) >> 6;
-      const bool far = B >= 0x100000000ull;
+      const bool near = t3 > t2 && B < (uint64_t) primes[t3 - 1u];
       uint32_t b = wave;
       uint32_t kN = t2 + (b << 6) + lane;
       uint32_t pN = (b < nb && kN < t3) ? primes[kN] : 0u;
@@ -333,19 +367,8 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         MN = (bn < nb && kN < t3) ? magics[kN] : 0ull;
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
-          const uint32_t a = p / 30u;
-          const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
-          const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
-#pragma unroll
-          for (uint32_t c = 0; c < 8; c++) {
-            const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
-            const uint32_t q = ent & 0xffu;
-            const uint32_t cc = a * q + (ent >> 8) - 1u;
-            uint32_t i = (cc >= R) ? cc - R : cc - R + p;
-            if (!far && q == 1u && B + i < (uint64_t) p)
-              i = 0xffffffffu;          // the prime itself; its next hit is beyond S
-            if (i < S) strike(sieve, i, c);
-          }
+          if (near) strike_big<true>(sieve, p, R, B, S, ctab);
+          else strike_big<false>(sieve, p, R, B, S, ctab);
         }
       }
     }
@@ -355,11 +378,11 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   // Entries are bit indices (byte * 8 + bit); four independent loads per
   // thread are in flight before their strikes.
   if (args.buckets) {
-    const uint32_t* __restrict__ offs = (const uint32_t*) args.listOff;
-    const uint32_t* __restrict__ ent = (const uint32_t*) args.buckets;
+    const auto* __restrict__ offs = gptr<uint32_t>(args.listOff);
+    const auto* __restrict__ ent = gptr<uint32_t>(args.buckets);
     const uint32_t e0 = offs[seg];
     const uint32_t room = offs[seg + 1u] - e0;
-    const uint32_t e1 = e0 + min(((const uint32_t*) args.listCount)[seg], room);
+    const uint32_t e1 = e0 + min(gptr<uint32_t>(args.listCount)[seg], room);
     uint32_t e = e0 + tid;
     for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
       const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
@@ -461,8 +484,8 @@ __device__ __forceinline__ uint32_t bucket_walk_round(const PsgBucketArgs& a, co
                                                       const uint16_t* ctab, uint32_t limit,
                                                       Emit emit)
 {
-  const uint32_t* __restrict__ primes = (const uint32_t*) a.primes;
-  const uint64_t* __restrict__ magics = (const uint64_t*) a.magics;
+  const auto* __restrict__ primes = gptr<uint32_t>(a.primes);
+  const auto* __restrict__ magics = gptr<uint64_t>(a.magics);
   const uint32_t segMask = (1u << a.segShift) - 1u;
   const uint32_t bytes = w.segs << a.segShift;
   const uint64_t V = 30ull * bytes;               // values covered by the window
@@ -519,7 +542,7 @@ __device__ __forceinline__ uint32_t bucket_walk_round(const PsgBucketArgs& a, co
 
 __device__ __forceinline__ const PsgWindow& window_of_wg(const PsgBucketArgs& a, uint32_t& chunk)
 {
-  const PsgWindow* win = (const PsgWindow*) a.windows;
+  const auto* win = gptr<PsgWindow>(a.windows);
   const uint32_t w = last_le(a.nWindows, blockIdx.x, [&](uint32_t i) { return win[i].wgBase; });
   chunk = blockIdx.x - win[w].wgBase;
   return win[w];
@@ -537,7 +560,7 @@ psg_sieve_count(PsgSieveArgs args)
   __shared__ uint32_t wsum[PSG_NCOUNTS];
   __shared__ uint32_t sSeg, sRun;
 
-  const PsgRun* runs = (const PsgRun*) args.runs;
+  const auto* runs = gptr<PsgRun>(args.runs);
   init_ctab(ctab);
   for (;;) {
     if (threadIdx.x == 0) {
@@ -585,16 +608,16 @@ psg_bucket_fill(PsgBucketArgs a)
   uint32_t c;
   const PsgWindow w = window_of_wg(a, c);
   const uint32_t W = w.segs;                           // <= PSG_MAX_WINDOW_SEGS <= blockDim
-  const uint32_t* __restrict__ listOff = (const uint32_t*) a.listOff + w.firstSeg;
+  const auto* __restrict__ listOff = gptr<uint32_t>(a.listOff) + w.firstSeg;
   uint32_t* listCount = (uint32_t*) a.listCount + w.firstSeg;
-  uint32_t* entries = (uint32_t*) a.entries;
+  auto* entries = gptr_rw<uint32_t>(a.entries);
   const uint32_t lane = tid & 63u, wave = tid >> 6, nwaves = nthreads >> 6;
   for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
 
   BucketWalk st;
   st.k = a.bigBegin + c * nthreads + tid;
-  st.pn = (st.k < w.bigEnd) ? ((const uint32_t*) a.primes)[st.k] : 0u;
-  st.Mn = (st.k < w.bigEnd) ? ((const uint64_t*) a.magics)[st.k] : 0ull;
+  st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
+  st.Mn = (st.k < w.bigEnd) ? gptr<uint64_t>(a.magics)[st.k] : 0ull;
   st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0ull;
   const uint32_t stride = nthreads * w.nChunks;
   uint64_t appended = 0;
