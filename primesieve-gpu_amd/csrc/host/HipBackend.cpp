@@ -190,7 +190,7 @@ uint32_t HipBackend::persistentGrid(uint32_t nSegs)
   if (!persistentWGs_)
   {
     int perCU = 0;
-    if (api_->ModuleOccupancy(&perCU, fnSieve_, (int) wgSize_, sieveBytes_) != hipSuccess || perCU < 1)
+    if (api_->ModuleOccupancy(&perCU, fnSieve_, (int) wgSize_, sieveBytes_ + PSG_SIEVE_LDS_TAIL) != hipSuccess || perCU < 1)
       perCU = 1;
     persistentWGs_ = (uint32_t) perCU * (dev_.computeUnits ? dev_.computeUnits : 256);
   }
@@ -445,7 +445,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
     fillPhase.push_back(false);
     void* pa[] = { &args };
     hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, persistentGrid(sl.nSegs), 1, 1, wgSize_, 1, 1,
-                                            sieveBytes_, stream_, pa, nullptr),
+                                            sieveBytes_ + PSG_SIEVE_LDS_TAIL, stream_, pa, nullptr),
              "hipModuleLaunchKernel(psg_sieve_count)");
   }
   recordPhase(nev++);

@@ -21,6 +21,7 @@
 
 #define PSG_NCOUNTS 6          /* primes, twins, triplets, ..., sextuplets */
 #define PSG_MAX_SEG_BYTES (160u * 1024u)
+#define PSG_SIEVE_LDS_TAIL 512u  /* LDS after the segment: class table, counters */
 #define PSG_TILE0_PERIOD 17017u  /* 7*11*13*17 bytes */
 #define PSG_TILE1_PERIOD 437u    /* 19*23 bytes */
 #define PSG_FIRST_CROSS_PRIME 29u

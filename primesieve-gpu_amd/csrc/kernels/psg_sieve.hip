@@ -58,6 +58,11 @@ __device__ __forceinline__ uint32_t mod_magic(uint64_t B, uint32_t p, uint64_t M
   return (uint32_t) B - (uint32_t) q * p;
 }
 
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
+{
+  return (x << r) | (x >> (32u - r));
+}
+
 // Start (byte index relative to segment base B) of residue class j of prime
 // p, given R = B mod p.  ctab[(p % 30) * 8 + j] packs q_j (the cofactor
 // residue mod 30 of class j) in bits 0..7 and e_j + 1 in bits 8..15, where
@@ -79,6 +84,43 @@ __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t
     i = (ii > 0xffffffffull) ? 0xffffffffu : (uint32_t) ii;
   }
   return i;
+}
+
+// All hits of a prime 256 < p <= S in the segment at B (R = B mod p).  Each
+// residue class is an arithmetic progression i, i+p, i+2p, ... of bytes; it
+// is struck as four progressions with stride 4p (a multiple of 4 bytes), so
+// each keeps one word mask: the masks of consecutive hits differ by a
+// rotation of 8 * (p mod 4) bits.
+template <bool kNear>
+__device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint32_t R, uint64_t B,
+                                              uint32_t S, const uint16_t* ctab)
+{
+  const uint32_t a = p / 30u;
+  const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
+  const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
+  const uint32_t step = 4u * p;
+  const uint32_t rot = (p & 3u) << 3;            // 8 or 24: p is odd
+#pragma unroll
+  for (uint32_t c = 0; c < 8; c++) {
+    const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
+    const uint32_t q = ent & 0xffu;
+    const uint32_t cc = a * q + (ent >> 8) - 1u;
+    uint32_t i = (cc >= R) ? cc - R : cc - R + p;
+    if (kNear && q == 1u && B + i < (uint64_t) p)
+      i += p;                                   // skip the prime itself
+    const uint32_t m0 = ~(1u << (((i & 3u) << 3) + c));
+    const uint32_t m1 = rotl32(m0, rot), m2 = rotl32(m1, rot), m3 = rotl32(m2, rot);
+    uint32_t w0 = i & ~3u, w1 = (i + p) & ~3u, w2 = (i + 2u * p) & ~3u, w3 = (i + 3u * p) & ~3u;
+    for (; w3 < S; w0 += step, w1 += step, w2 += step, w3 += step) {
+      atomicAnd(&sieve[w0 >> 2], m0);
+      atomicAnd(&sieve[w1 >> 2], m1);
+      atomicAnd(&sieve[w2 >> 2], m2);
+      atomicAnd(&sieve[w3 >> 2], m3);
+    }
+    if (w0 < S) atomicAnd(&sieve[w0 >> 2], m0);
+    if (w1 < S) atomicAnd(&sieve[w1 >> 2], m1);
+    if (w2 < S) atomicAnd(&sieve[w2 >> 2], m2);
+  }
 }
 
 // The <= 8 hits of a prime S < p in the segment at B (R = B mod p): one per
@@ -106,11 +148,6 @@ __device__ __forceinline__ void strike_big(uint32_t* sieve, uint32_t p, uint32_t
 __device__ __forceinline__ void strike_bit(uint32_t* sieve, uint32_t v)
 {
   atomicAnd(&sieve[v >> 5], ~(1u << (v & 31u)));
-}
-
-__device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
-{
-  return (x << r) | (x >> (32u - r));
 }
 
 // Strike one class progression from byte `pos` with word stride `step`
@@ -284,66 +321,26 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
       }
     }
 
-    // tier 1 (256 < p <= S/8): batches of 8 primes; lane (slot, class)
-    // computes one class start, then the whole wave strikes each prime with
-    // lane = (class, hit) and stride 8p (loop-invariant word mask).
+    // tiers 1 and 2 (256 < p <= S): lane = prime, 64 consecutive primes per
+    // wave; each class of p is struck as four interleaved progressions with
+    // stride 4p, whose word masks are loop-invariant.
     {
-      const uint32_t nb = (t1 - t0 + 7u) >> 3;
+      const uint32_t nb = (t2 - t0 + 63u) >> 6;
       uint32_t b = wave;
-      uint32_t kN = t0 + b * 8u + slot;
-      uint32_t pN = (b < nb && kN < t1) ? primes[kN] : 0u;
-      uint64_t MN = (b < nb && kN < t1) ? magics[kN] : 0ull;
-      for (; b < nb; b += nwaves) {
-        const uint32_t p = pN;
-        const uint64_t M = MN;
-        const uint32_t bn = b + nwaves;
-        kN = t0 + bn * 8u + slot;
-        pN = (bn < nb && kN < t1) ? primes[kN] : 0u;
-        MN = (bn < nb && kN < t1) ? magics[kN] : 0ull;
-        const uint32_t R = p ? mod_magic(B, p, M) : 0u;
-        const uint32_t st = !p ? 0u : nearSmall ? class_start<true>(p, R, B, j, ctab)
-                                               : class_start<false>(p, R, B, j, ctab);
-        uint32_t sq[8];
-#pragma unroll
-        for (uint32_t q = 0; q < 8; q++) sq[q] = (uint32_t) __shfl((int) st, (int) (q * 8u + j));
-#pragma unroll
-        for (uint32_t q = 0; q < 8; q++) {
-          const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
-          if (!ps) break;
-          strike_run(sieve, sq[q] + slot * ps, 8u * ps, S, j);
-        }
-      }
-    }
-
-    // tier 2 (S/8 < p <= S): lane = (prime slot, class), stride p; the word
-    // mask rotates by 8 * (p mod 4) bits per step.
-    {
-      const uint32_t nb = (t2 - t1 + 7u) >> 3;
-      uint32_t b = wave;
-      uint32_t kN = t1 + b * 8u + slot;
+      uint32_t kN = t0 + (b << 6) + lane;
       uint32_t pN = (b < nb && kN < t2) ? primes[kN] : 0u;
       uint64_t MN = (b < nb && kN < t2) ? magics[kN] : 0ull;
       for (; b < nb; b += nwaves) {
         const uint32_t p = pN;
         const uint64_t M = MN;
         const uint32_t bn = b + nwaves;
-        kN = t1 + bn * 8u + slot;
+        kN = t0 + (bn << 6) + lane;
         pN = (bn < nb && kN < t2) ? primes[kN] : 0u;
         MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
-          uint32_t i = nearSmall ? class_start<true>(p, R, B, j, ctab)
-                                 : class_start<false>(p, R, B, j, ctab);
-          // two strikes per trip: the second mask is the first rotated by
-          // 8 * (p mod 4); after two steps the rotation is always 16 bits
-          uint32_t mask = ~(1u << (((i & 3u) << 3) + j));
-          const uint32_t rot = (p & 3u) << 3;
-          for (; i + p < S; i += 2u * p) {
-            atomicAnd(&sieve[i >> 2], mask);
-            atomicAnd(&sieve[(i + p) >> 2], rotl32(mask, rot));
-            mask = rotl32(mask, 16u);
-          }
-          if (i < S) atomicAnd(&sieve[i >> 2], mask);
+          if (nearSmall) strike_medium<true>(sieve, p, R, B, S, ctab);
+          else strike_medium<false>(sieve, p, R, B, S, ctab);
         }
       }
     }
@@ -555,10 +552,14 @@ __device__ __forceinline__ const PsgWindow& window_of_wg(const PsgBucketArgs& a,
 extern "C" __global__ void __launch_bounds__(1024)
 psg_sieve_count(PsgSieveArgs args)
 {
-  extern __shared__ uint32_t sieve[];
-  __shared__ __attribute__((aligned(16))) uint16_t ctab[240];
-  __shared__ uint32_t wsum[PSG_NCOUNTS];
-  __shared__ uint32_t sSeg, sRun;
+  // the sieve starts at LDS address 0 (strike addresses need no base); the
+  // class table and the block's small variables follow it
+  extern __shared__ uint32_t lds[];
+  uint32_t* sieve = lds;
+  uint16_t* ctab = (uint16_t*) (lds + (args.segBytes >> 2));   // 240 entries, 16-byte aligned
+  uint32_t* wsum = lds + (args.segBytes >> 2) + 120u;          // PSG_NCOUNTS
+  uint32_t& sSeg = wsum[PSG_NCOUNTS];
+  uint32_t& sRun = wsum[PSG_NCOUNTS + 1];
 
   const auto* runs = gptr<PsgRun>(args.runs);
   init_ctab(ctab);
