@@ -13,7 +13,8 @@ Prints ONE JSON line:
                 roofline: achieved = algorithmic LDS bytes per launch / average
                 launch duration (HIP events on the backend's stream), with
                 algorithmic LDS bytes = 8*C + 8*W (C = crossings by sieving
-                primes >= 29: one 4-byte read + one 4-byte write each; W =
+                primes >= 53 -- 7..47 are pre-sieve tiles: one 4-byte read
+                + one 4-byte write each; W =
                 32-bit words of the bit array: one fill write + one count
                 read); peak = the same 8-bytes-per-ds_and_b32 figure measured
                 by the conflict-free LDS microbenchmark on the same device;

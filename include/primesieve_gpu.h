@@ -54,7 +54,7 @@ typedef struct psg_tuning {
   uint32_t segments_per_launch;
   uint32_t slab_segments;     /* segments per big-prime bucket slab */
   uint32_t bucket_factor;     /* primes > bucket_factor * sieve_bytes use buckets */
-  uint32_t presieve;          /* 1 = pre-sieve tiles for 7..23 */
+  uint32_t presieve;          /* 1 = pre-sieve tiles for 7..47 */
   uint32_t bucket_margin;     /* room of a segment's bucket list beyond the expected
                                  count, in standard deviations; 0 = auto (6) */
 } psg_tuning;
@@ -121,21 +121,22 @@ typedef struct psg_stats {
   uint32_t launches;         /* sieve kernel launches */
   uint32_t sieve_bytes;      /* LDS bytes per segment */
   uint32_t work_group_size;  /* threads per workgroup */
-  uint32_t sieving_primes;   /* primes >= 29 with p*p <= stop */
+  uint32_t sieving_primes;   /* sieving primes struck in LDS (from 53; from 7 without pre-sieve) with p*p <= stop */
   uint32_t bucket_retries;   /* calls repeated because a bucket list outgrew its room */
   uint32_t reserved;
 } psg_stats;
 int psg_gpu_last_stats(psg_stats* out);
 
 /* Algorithmic work of sieving [start, stop] with segments of seg_bytes:
- * crossings by sieving primes p >= 29 (all tiers), the 32-bit words of the
+ * crossings by sieving primes p >= 53 (the primes struck in LDS; 7..47 are
+ * removed by the pre-sieve tiles), the 32-bit words of the
  * bit array, and the crossings by primes above 8*seg_bytes ("big" primes).
  * Closed form on the host; no device needed. */
 typedef struct psg_work {
-  uint64_t crossings;        /* C: sum over 29 <= p <= isqrt(stop) */
+  uint64_t crossings;        /* C: sum over 53 <= p <= isqrt(stop) */
   uint64_t words32;          /* W = ceil(bytes / 4) */
   uint64_t crossings_big;    /* crossings by p > 8 * seg_bytes */
-  uint64_t sieving_primes;   /* primes in [29, isqrt(stop)] */
+  uint64_t sieving_primes;   /* primes in [53, isqrt(stop)] */
 } psg_work;
 int psg_algorithmic_work(uint64_t start, uint64_t stop, uint32_t seg_bytes, psg_work* out);
 

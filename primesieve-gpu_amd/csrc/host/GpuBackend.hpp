@@ -42,7 +42,7 @@ struct GpuConfig
   uint32_t sieveBytes = 0;         // LDS bytes per segment (multiple of 256)
   uint32_t workGroupSize = 0;      // threads per workgroup, multiple of 64
   uint32_t segmentsPerLaunch = 0;  // segments per kernel launch
-  bool preSieve = true;            // pre-sieve tiles for 7..23
+  bool preSieve = true;            // pre-sieve tiles for 7..47
   uint32_t slabSegments = 0;       // segments per big-prime bucket slab
   uint32_t bucketFactor = 0;       // primes > bucketFactor * sieveBytes use the
                                    // bucket tier; 0 = auto (8), ~0u = never

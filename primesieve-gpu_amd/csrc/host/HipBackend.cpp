@@ -64,8 +64,7 @@ HipBackend::~HipBackend()
   (void) api_->SetDevice(dev_.deviceIndex);
   primes_.release();
   magics_.release();
-  tile0_.release();
-  tile1_.release();
+  tiles_.release();
   counts_.release();
   descriptors_.release();
   workCounters_.release();
@@ -86,6 +85,7 @@ void HipBackend::init()
 
   // geometry: LDS segment, workgroup, launch size
   const uint64_t ldsMax = std::min<uint64_t>(dev_.localMemBytes, PSG_MAX_SEG_BYTES + 1024);
+  ldsBytes_ = (uint32_t) ldsMax;
   uint32_t sb = config_.sieveBytes ? config_.sieveBytes : 128u * 1024u;
   sb = (sb / 256u) * 256u;
   if (sb < 256u) sb = 256u;
@@ -127,14 +127,27 @@ void HipBackend::init()
 
   if (config_.preSieve)
   {
-    const std::vector<uint32_t> t0 = tileWindows({ 7, 11, 13, 17 });
-    const std::vector<uint32_t> t1 = tileWindows({ 19, 23 });
-    if (t0.size() != PSG_TILE0_PERIOD || t1.size() != PSG_TILE1_PERIOD)
-      throw primesieve_error("pre-sieve tile period mismatch");
-    tile0_.allocate(api_, t0.size() * 4);
-    tile1_.allocate(api_, t1.size() * 4);
-    hipCheck(api_, api_->Memcpy(tile0_.get(), t0.data(), t0.size() * 4, hipMemcpyHostToDevice), "hipMemcpy tile0");
-    hipCheck(api_, api_->Memcpy(tile1_.get(), t1.data(), t1.size() * 4, hipMemcpyHostToDevice), "hipMemcpy tile1");
+    const std::vector<std::vector<uint32_t>> groups = { { 7, 11, 13, 17 }, { 19, 23 }, { 29, 31 },
+                                                        { 37, 41 }, { 43, 47 } };
+    const uint32_t periods[PSG_NTILES] = PSG_TILE_PERIODS;
+    const uint32_t rows[PSG_NTILES] = PSG_TILE_ROWS;
+    const uint32_t offsets[PSG_NTILES] = PSG_TILE_OFFSETS;
+    std::vector<uint32_t> all;
+    for (size_t t = 0; t < groups.size(); t++)
+    {
+      const std::vector<uint32_t> w = tileWindows(groups[t]);
+      const uint32_t Q = rows[t];
+      if (w.size() != periods[t] || all.size() != offsets[t] || Q != (periods[t] + 3) / 4)
+        throw primesieve_error("pre-sieve tile layout mismatch");
+      std::vector<uint32_t> tile((size_t) 4 * Q, 0);
+      for (uint32_t i = 0; i < periods[t]; i++)
+        tile[(size_t) (i % 4) * Q + i / 4] = w[i];
+      all.insert(all.end(), tile.begin(), tile.end());
+    }
+    if (all.size() != PSG_TILE_WORDS)
+      throw primesieve_error("pre-sieve tile layout mismatch");
+    tiles_.allocate(api_, all.size() * 4);
+    hipCheck(api_, api_->Memcpy(tiles_.get(), all.data(), all.size() * 4, hipMemcpyHostToDevice), "hipMemcpy tiles");
   }
   void* hc = nullptr;
   hipCheck(api_, api_->HostMalloc(&hc, 16, 0), "hipHostMalloc");
@@ -190,7 +203,7 @@ uint32_t HipBackend::persistentGrid(uint32_t nSegs)
   if (!persistentWGs_)
   {
     int perCU = 0;
-    if (api_->ModuleOccupancy(&perCU, fnSieve_, (int) wgSize_, sieveBytes_ + PSG_SIEVE_LDS_TAIL) != hipSuccess || perCU < 1)
+    if (api_->ModuleOccupancy(&perCU, fnSieve_, (int) wgSize_, sieveLds()) != hipSuccess || perCU < 1)
       perCU = 1;
     persistentWGs_ = (uint32_t) perCU * (dev_.computeUnits ? dev_.computeUnits : 256);
   }
@@ -390,8 +403,8 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   std::memset(&args, 0, sizeof args);
   args.primes = primes_.addr();
   args.magics = magics_.addr();
-  args.tile0 = tile0_.addr();
-  args.tile1 = tile1_.addr();
+  args.tiles = tiles_.addr();
+  args.tilesInLds = (tiles_.addr() && tilesFitLds()) ? 1u : 0u;
   args.counts = counts_.addr();
   args.segBytes = S;
   args.tier0End = primesUpTo(std::min<uint32_t>(PSG_TIER0_MAX_PRIME, S / 8));
@@ -445,7 +458,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
     fillPhase.push_back(false);
     void* pa[] = { &args };
     hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, persistentGrid(sl.nSegs), 1, 1, wgSize_, 1, 1,
-                                            sieveBytes_ + PSG_SIEVE_LDS_TAIL, stream_, pa, nullptr),
+                                            sieveLds(), stream_, pa, nullptr),
              "hipModuleLaunchKernel(psg_sieve_count)");
   }
   recordPhase(nev++);

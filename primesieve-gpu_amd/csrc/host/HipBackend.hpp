@@ -110,12 +110,24 @@ private:
   bool launchPlan(uint32_t outs, const std::vector<uint32_t>& room);
   uint32_t primesUpTo(uint64_t x) const;
   uint32_t persistentGrid(uint32_t nSegs);
+  // LDS of the segment kernel: the segment, its tail and (when they fit) the
+  // small pre-sieve tiles
+  bool tilesFitLds() const
+  {
+    return (uint64_t) sieveBytes_ + PSG_SIEVE_LDS_TAIL + 4ull * PSG_SMALL_TILE_WORDS <= ldsBytes_;
+  }
+  uint32_t sieveLds() const
+  {
+    return sieveBytes_ + PSG_SIEVE_LDS_TAIL
+         + ((config_.preSieve && tilesFitLds()) ? 4u * PSG_SMALL_TILE_WORDS : 0u);
+  }
   void recordPhase(size_t i);
 
   const HipApi* api_;
   primesieve::gpu::DeviceInfo dev_;
   primesieve::gpu::GpuConfig config_;
   uint32_t sieveBytes_ = 0;
+  uint32_t ldsBytes_ = 0;         // LDS per workgroup the device offers (capped)
   uint32_t wgSize_ = 0;
   uint32_t segmentsPerLaunch_ = 0;
   uint32_t bucketFactor_ = 8;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
@@ -134,8 +146,7 @@ private:
 
   DeviceBuffer primes_;        // uint32 sieving primes >= first cross prime
   DeviceBuffer magics_;        // uint64 division magics
-  DeviceBuffer tile0_;
-  DeviceBuffer tile1_;
+  DeviceBuffer tiles_;        // pre-sieve tiles
   DeviceBuffer counts_;        // uint64 [outputs][6]
   DeviceBuffer descriptors_;   // runs and windows of every slab of a call
   DeviceBuffer workCounters_;  // uint32 per slab

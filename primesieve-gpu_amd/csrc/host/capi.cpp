@@ -207,11 +207,11 @@ int psg_algorithmic_work(uint64_t start, uint64_t stop, uint32_t seg_bytes, psg_
       return 0;
     const psg::ByteRange br = psg::byteRange(lo, stop);
     const uint32_t sq = (uint32_t) psg::isqrt64(stop);
-    const std::vector<uint32_t> pr = psg::primesBetween(29, sq);
+    const std::vector<uint32_t> pr = psg::primesBetween(PSG_FIRST_CROSS_PRIME, sq);
     const uint64_t bytes = br.gHi - br.gLo + 1;
     out->words32 = (bytes + 3) / 4;
     out->sieving_primes = pr.size();
-    out->crossings = psg::crossings(pr, lo, stop, 29, sq);
+    out->crossings = psg::crossings(pr, lo, stop, PSG_FIRST_CROSS_PRIME, sq);
     const uint64_t big = 8ull * seg_bytes;
     out->crossings_big = big < sq ? psg::crossings(pr, lo, stop, (uint32_t) (big + 1), sq) : 0;
     return 0;

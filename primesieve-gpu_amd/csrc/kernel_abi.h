@@ -22,9 +22,20 @@
 #define PSG_NCOUNTS 6          /* primes, twins, triplets, ..., sextuplets */
 #define PSG_MAX_SEG_BYTES (160u * 1024u)
 #define PSG_SIEVE_LDS_TAIL 512u  /* LDS after the segment: class table, counters */
-#define PSG_TILE0_PERIOD 17017u  /* 7*11*13*17 bytes */
-#define PSG_TILE1_PERIOD 437u    /* 19*23 bytes */
-#define PSG_FIRST_CROSS_PRIME 29u
+/* Pre-sieve tiles: the bit pattern of each group of small primes repeats
+ * every P = product-of-the-group bytes.  A tile holds the P sliding 32-bit
+ * windows of its pattern (window i = pattern bytes i..i+3, wrapping) in
+ * residue-major order: window i at word (i % 4) * Q + i / 4, Q = ceil(P/4),
+ * so the windows 4 apart that consecutive sieve words need are adjacent.
+ * Tiles are stored one after another; tiles 1.. (PSG_SMALL_TILE_WORDS
+ * words) are copied into LDS when the segment leaves room. */
+#define PSG_NTILES 5
+#define PSG_TILE_PERIODS { 17017u, 437u, 899u, 1517u, 2021u }   /* 7*11*13*17, 19*23, 29*31, 37*41, 43*47 */
+#define PSG_TILE_ROWS { 4255u, 110u, 225u, 380u, 506u }         /* Q = ceil(P / 4) */
+#define PSG_TILE_OFFSETS { 0u, 17020u, 17460u, 18360u, 19880u } /* first word of each tile */
+#define PSG_TILE_WORDS 21904u
+#define PSG_SMALL_TILE_WORDS (PSG_TILE_WORDS - 17020u)
+#define PSG_FIRST_CROSS_PRIME 53u
 #define PSG_TIER0_MAX_PRIME 256u
 
 /* One run (48 bytes). */
@@ -46,8 +57,8 @@ typedef struct PsgSieveArgs {
   uint64_t runs;        /* const PsgRun*, sorted by segStart */
   uint64_t primes;      /* const uint32_t*: sieving primes, ascending */
   uint64_t magics;      /* const uint64_t*: ceil(2^(64+L)/p), L = floor(log2 p) */
-  uint64_t tile0;       /* const uint32_t*: sliding windows (0 = no pre-sieve) */
-  uint64_t tile1;       /* const uint32_t*: sliding windows */
+  uint64_t tiles;       /* const uint32_t[PSG_TILE_WORDS]: pre-sieve tiles (0 = no pre-sieve) */
+  uint64_t tilesInLds;  /* 1: the kernel keeps tiles 1.. in LDS after the segment and its tail */
   uint64_t counts;      /* uint64_t[outs][PSG_NCOUNTS], accumulated atomically */
   uint64_t buckets;     /* const uint32_t*: bucket entries of the slab (0 = none) */
   uint64_t listOff;     /* const uint32_t[nSegs + 1]: segment s owns entries [listOff[s], listOff[s+1]) */

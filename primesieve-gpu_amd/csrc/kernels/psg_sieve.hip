@@ -8,24 +8,25 @@
 // reference kernel ps_count (src/gpu/cuda/sieve.cu:157-320) is reorganised
 // for CDNA4:
 //
-//   1. fill:   two pre-sieve tiles remove the multiples of 7..23
-//              (periods 7*11*13*17 and 19*23 bytes, read as sliding 32-bit
-//              windows), then the tile primes' own bits are put back;
-//   2. strike: every sieving prime p >= 29 is split into its 8 residue
+//   1. fill:   five pre-sieve tiles remove the multiples of 7..47 (groups
+//              {7,11,13,17} {19,23} {29,31} {37,41} {43,47}; each pattern
+//              repeats every product-of-the-group bytes and is read as
+//              sliding 32-bit windows), then the tile primes' own bits are
+//              put back;
+//   2. strike: every sieving prime p >= 53 is split into its 8 residue
 //              classes.  In class j the multiples p*q (q coprime to 30) lie at
 //              global bytes c_j + t*p, so a class is an arithmetic progression
 //              with byte stride p and a constant bit -- no wheel state
 //              machine and no per-crossing table reads.  Tiers:
 //                tier 0 (p <= 256): each wave strikes all of them over its
-//                        own slice of the segment (even load);
-//                tier 1 (p <= S/8): batches of 8 primes per wave; lane =
+//                        own slice of the segment (even load); lane =
 //                        (class, hit), stride 8p, loop-invariant word mask;
-//                tier 2 (S/8 < p <= S): lane = (prime, class), stride p with
-//                        a rotating mask;
+//                tiers 1-2 (256 < p <= S): lane = prime; each class as four
+//                        progressions of stride 4p with loop-invariant masks;
 //                tier 3 (S < p <= bucket threshold): lane = prime, at most
 //                        one hit per class;
-//              plus the bucket lists of psg_bucket_fill for larger primes
-//              (one LDS atomic per entry);
+//              plus the bucket list of the segment, filled by
+//              psg_bucket_fill for larger primes (one LDS atomic per entry);
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
 //              pass; wave sums use ds_swizzle butterflies.
 //
@@ -77,7 +78,7 @@ __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t
   const uint32_t b = p - a * 30u;
   const uint32_t ent = ctab[b * 8u + j];
   const uint32_t q = ent & 0xffu;
-  const uint32_t c = a * q + (ent >> 8) - 1u;
+  const uint32_t c = __umul24(a, q) + (ent >> 8) - 1u;     // a < 2^24 for every caller
   uint32_t i = (c >= R) ? c - R : c - R + p;
   if (kNear && q == 1u && B + i < (uint64_t) p) {
     const uint64_t ii = (uint64_t) i + p;
@@ -104,7 +105,7 @@ __device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint3
   for (uint32_t c = 0; c < 8; c++) {
     const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
     const uint32_t q = ent & 0xffu;
-    const uint32_t cc = a * q + (ent >> 8) - 1u;
+    const uint32_t cc = __umul24(a, q) + (ent >> 8) - 1u;    // p <= S: a < 2^24
     uint32_t i = (cc >= R) ? cc - R : cc - R + p;
     if (kNear && q == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
@@ -125,7 +126,8 @@ __device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint3
 
 // The <= 8 hits of a prime S < p in the segment at B (R = B mod p): one per
 // residue class at most.  The class table row of p is one 16-byte LDS read.
-template <bool kNear>
+// kSmall: p < 30 * 2^24, so that a * q is a 24-bit multiply.
+template <bool kNear, bool kSmall>
 __device__ __forceinline__ void strike_big(uint32_t* sieve, uint32_t p, uint32_t R, uint64_t B,
                                            uint32_t S, const uint16_t* ctab)
 {
@@ -136,7 +138,7 @@ __device__ __forceinline__ void strike_big(uint32_t* sieve, uint32_t p, uint32_t
   for (uint32_t c = 0; c < 8; c++) {
     const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
     const uint32_t q = ent & 0xffu;
-    const uint32_t cc = a * q + (ent >> 8) - 1u;
+    const uint32_t cc = (kSmall ? __umul24(a, q) : a * q) + (ent >> 8) - 1u;
     uint32_t i = (cc >= R) ? cc - R : cc - R + p;
     if (kNear && q == 1u && B + i < (uint64_t) p)
       i = 0xffffffffu;                  // the prime itself; its next hit is beyond S
@@ -222,11 +224,53 @@ __device__ void init_ctab(uint16_t* ctab)
   }
 }
 
+// Fill the segment from the pre-sieve tiles: word w is the AND over the
+// tiles of window (B + 4w) mod P.  Tile 0 comes from global memory, the small
+// tiles from their LDS copy (kLds) or global memory.  Eight words per thread
+// per trip, so that all their loads are in flight before the first store.
+template <bool kLds>
+__device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint64_t B,
+                                           const PSG_GLOBAL uint32_t* tiles, const uint32_t* small)
+{
+  constexpr uint32_t kPeriod[PSG_NTILES] = PSG_TILE_PERIODS;
+  constexpr uint32_t kRows[PSG_NTILES] = PSG_TILE_ROWS;
+  constexpr uint32_t kOffset[PSG_NTILES] = PSG_TILE_OFFSETS;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthreads = blockDim.x;
+  uint32_t idx[PSG_NTILES], stp[PSG_NTILES];
+#pragma unroll
+  for (int t = 0; t < PSG_NTILES; t++) {
+    idx[t] = (uint32_t) ((B + 4ull * tid) % kPeriod[t]);
+    stp[t] = (4u * nthreads) % kPeriod[t];
+  }
+  const auto word = [&](uint32_t w) {
+    uint32_t v = 0xffffffffu;
+#pragma unroll
+    for (int t = 0; t < PSG_NTILES; t++) {
+      const uint32_t pos = (idx[t] & 3u) * kRows[t] + (idx[t] >> 2);
+      v &= (t == 0 || !kLds) ? tiles[kOffset[t] + pos] : small[kOffset[t] - kOffset[1] + pos];
+      idx[t] += stp[t];
+      idx[t] = min(idx[t], idx[t] - kPeriod[t]);     // below 0 wraps to a large value
+    }
+    return v;
+  };
+  uint32_t w = tid;
+  for (; w + 7u * nthreads < words; w += 8u * nthreads) {
+    uint32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = word(w + u * nthreads);
+#pragma unroll
+    for (int u = 0; u < 8; u++) sieve[w + u * nthreads] = v[u];
+  }
+  for (; w < words; w += nthreads) sieve[w] = word(w);
+}
+
 // One segment: fill, strike, count.  Ends with a barrier, so that the LDS
 // sieve and the wave sums may be reused by the next segment.
 __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const PsgRun& run,
                                               uint32_t seg, uint32_t* sieve,
-                                              const uint16_t* ctab, uint32_t* wsum)
+                                              const uint16_t* ctab, uint32_t* wsum,
+                                              const uint32_t* small)
 {
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
@@ -237,45 +281,22 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
 
   const auto* __restrict__ primes = gptr<uint32_t>(args.primes);
   const auto* __restrict__ magics = gptr<uint64_t>(args.magics);
-  const auto* __restrict__ tile0 = gptr<uint32_t>(args.tile0);
-  const auto* __restrict__ tile1 = gptr<uint32_t>(args.tile1);
+  const auto* __restrict__ tiles = gptr<uint32_t>(args.tiles);
 
   // ---- 1. fill from the pre-sieve tiles ----------------------------------
-  if (!tile0) {
+  if (!tiles) {
     // pre-sieving disabled: every prime >= 7 is struck in step 2
     for (uint32_t w = tid; w < words; w += nthreads)
       sieve[w] = 0xffffffffu;
   } else {
-    uint32_t i0 = (uint32_t) ((B + 4ull * tid) % PSG_TILE0_PERIOD);
-    uint32_t i1 = (uint32_t) ((B + 4ull * tid) % PSG_TILE1_PERIOD);
-    const uint32_t s0 = (4u * nthreads) % PSG_TILE0_PERIOD;
-    const uint32_t s1 = (4u * nthreads) % PSG_TILE1_PERIOD;
-    // eight words per trip: all sixteen tile loads are issued before the
-    // first LDS store waits on them
-    uint32_t w = tid;
-    for (; w + 7u * nthreads < words; w += 8u * nthreads) {
-      uint32_t a0[8], a1[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        a0[u] = i0; a1[u] = i1;
-        i0 += s0; if (i0 >= PSG_TILE0_PERIOD) i0 -= PSG_TILE0_PERIOD;
-        i1 += s1; if (i1 >= PSG_TILE1_PERIOD) i1 -= PSG_TILE1_PERIOD;
-      }
-      uint32_t v[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = tile0[a0[u]] & tile1[a1[u]];
-#pragma unroll
-      for (int u = 0; u < 8; u++) sieve[w + u * nthreads] = v[u];
-    }
-    for (; w < words; w += nthreads) {
-      sieve[w] = tile0[i0] & tile1[i1];
-      i0 += s0; if (i0 >= PSG_TILE0_PERIOD) i0 -= PSG_TILE0_PERIOD;
-      i1 += s1; if (i1 >= PSG_TILE1_PERIOD) i1 -= PSG_TILE1_PERIOD;
-    }
-    // the tiles also cleared 7..23 themselves: global byte 0 is written by
-    // thread 0 above, so thread 0 restores it without a barrier
-    if (B == 0 && tid == 0)
-      sieve[0] |= 0x3fu;
+    if (args.tilesInLds) fill_tiles<true>(sieve, words, B, tiles, small);
+    else fill_tiles<false>(sieve, words, B, tiles, small);
+    // the tiles also cleared the tile primes 7..47 themselves: all of global
+    // byte 0 (7..31) and bits 0-3 of global byte 1 (37..47).  A segment
+    // that starts at byte 0 or 1 has them in its word 0, which thread 0
+    // wrote above and restores without a barrier.
+    if (B < 2u && tid == 0)
+      sieve[0] |= (B == 0) ? 0x0fffu : 0x000fu;
   }
   __syncthreads();
 
@@ -350,7 +371,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     // against the largest tier-3 prime).
     {
       const uint32_t nb = (t3 - t2 + 63u) >> 6;
-      const bool near = t3 > t2 && B < (uint64_t) primes[t3 - 1u];
+      const uint32_t pTop = (t3 > t2) ? primes[t3 - 1u] : 0u;
+      const bool near = B < (uint64_t) pTop;
+      const bool small = pTop < (30u << 24);
       uint32_t b = wave;
       uint32_t kN = t2 + (b << 6) + lane;
       uint32_t pN = (b < nb && kN < t3) ? primes[kN] : 0u;
@@ -364,8 +387,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         MN = (bn < nb && kN < t3) ? magics[kN] : 0ull;
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
-          if (near) strike_big<true>(sieve, p, R, B, S, ctab);
-          else strike_big<false>(sieve, p, R, B, S, ctab);
+          if (near) strike_big<true, false>(sieve, p, R, B, S, ctab);
+          else if (small) strike_big<false, true>(sieve, p, R, B, S, ctab);
+          else strike_big<false, false>(sieve, p, R, B, S, ctab);
         }
       }
     }
@@ -560,6 +584,14 @@ psg_sieve_count(PsgSieveArgs args)
   uint32_t* wsum = lds + (args.segBytes >> 2) + 120u;          // PSG_NCOUNTS
   uint32_t& sSeg = wsum[PSG_NCOUNTS];
   uint32_t& sRun = wsum[PSG_NCOUNTS + 1];
+  // the small pre-sieve tiles, copied once per workgroup when they fit
+  uint32_t* small = lds + ((args.segBytes + PSG_SIEVE_LDS_TAIL) >> 2);
+  if (args.tilesInLds) {
+    constexpr uint32_t kOffset[PSG_NTILES] = PSG_TILE_OFFSETS;
+    const auto* tiles = gptr<uint32_t>(args.tiles);
+    for (uint32_t i = threadIdx.x; i < PSG_SMALL_TILE_WORDS; i += blockDim.x)
+      small[i] = tiles[kOffset[1] + i];
+  }
 
   const auto* runs = gptr<PsgRun>(args.runs);
   init_ctab(ctab);
@@ -575,7 +607,7 @@ psg_sieve_count(PsgSieveArgs args)
     const uint32_t seg = sSeg;
     if (seg >= args.nSegs) break;
     const PsgRun run = runs[sRun];
-    sieve_segment(args, run, seg, sieve, ctab, wsum);
+    sieve_segment(args, run, seg, sieve, ctab, wsum, small);
   }
 }
 

@@ -33,11 +33,11 @@ def test_algorithmic_work_matches_oracle(psgpu, oracle):
     for a, b in [(0, 10 ** 6), (10 ** 9, 10 ** 9 + 10 ** 6), (12345, 98765)]:
         w = psgpu.algorithmic_work(a, b, 65536)
         lo = max(a, 7)
-        assert w.crossings == oracle.crossings(lo, b, 29, 2 ** 32 - 1)
+        assert w.crossings == oracle.crossings(lo, b, 53, 2 ** 32 - 1)
         assert w.crossings_big == oracle.crossings(lo, b, 8 * 65536 + 1, 2 ** 32 - 1)
     w = psgpu.algorithmic_work(0, 10 ** 10, 65536)
     assert w.words32 == 83333334          # ceil(bytes / 4) of [7, 1e10]
-    assert w.crossings == 3095675604      # crossings by primes >= 29
+    assert w.crossings == 2661834318      # crossings by primes >= 53 (oracle)
 
 
 @pytest.fixture(scope="module")
