@@ -193,13 +193,26 @@ def choose(model, queries, grid=GRID):
     return pred.argmin(1), pred
 
 
-def choose_for_batch(model, queries, grid=GRID):
-    """One tuning for a whole batch: the grid point with the smallest sum of
-    predicted times; and the queries ordered by predicted time, largest first
-    (the packing order handed to the persistent engine)."""
-    pred = model.predict(design_matrix(queries, grid)).reshape(len(queries), len(grid))
-    tot = np.exp2(pred).sum(0)
-    g = int(tot.argmin())
+def choose_for_batch(model, queries, grid=GRID, min_gain=0.05):
+    """One tuning for a whole batch, and the packing order of its queries.
+
+    The model predicts the device time of one count call, which includes a
+    per-call cost (launches, the sieving-prime set-up of the call) that a
+    persistent batch pays once.  So a query adds to the batch its predicted
+    time minus the predicted time of the shortest interval (2^16) at the same
+    start; the grid point with the smallest sum wins, but the default
+    geometry is kept unless the winner is predicted min_gain faster.  The
+    queries are ordered by predicted time, largest first (the order handed to
+    the persistent engine)."""
+    n, G = len(queries), len(grid)
+    pred = model.predict(design_matrix(queries, grid)).reshape(n, G)
+    base = model.predict(design_matrix([(a, 1 << 16, k) for a, _, k in queries], grid)).reshape(n, G)
+    marginal = np.maximum(np.exp2(pred) - np.exp2(base), 0.0).sum(0)
+    g = int(marginal.argmin())
+    if DEFAULT in grid:
+        d = grid.index(DEFAULT)
+        if marginal[g] > (1.0 - min_gain) * marginal[d]:
+            g = d
     order = np.argsort(-pred[:, g], kind="stable")
     return grid[g], order
 
