@@ -212,6 +212,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
+    ap.add_argument("--only", choices=["pi_1e10", "cost_model", "batched_intervals", "serving"],
+                    help="run just this workload (no headline) and print its JSON, e.g. under a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the count path runs on a single MI355X; "
@@ -221,6 +223,15 @@ def main():
     import psgpu as m
     if not m.available():
         raise SystemExit("no usable GPU: " + m.last_error())
+    if args.only:
+        model = None
+        if args.only in ("cost_model", "batched_intervals", "serving"):
+            model, cm = workload_costmodel()
+        res = {"pi_1e10": lambda: workload_pi(m), "cost_model": lambda: cm,
+               "batched_intervals": lambda: workload_batch(m, model),
+               "serving": lambda: workload_serving(m, model)}[args.only]()
+        print(json.dumps({"workload": args.only, "result": res}), flush=True)
+        return
 
     got, dt, st, avg_launch_ms, launches = headline(m, args.steps, args.warmup)
     value = (STOP - START + 1) * args.steps / dt
