@@ -83,7 +83,7 @@ typedef struct PsgSieveArgs {
  * that ends above its room raises *overflow and the host repeats the call
  * with rooms equal to the counted totals. */
 #define PSG_BUCKET_THREADS 256u      /* threads of a psg_bucket_fill workgroup */
-#define PSG_BUCKET_HITS 32u          /* hits a fill thread stages per round */
+#define PSG_BUCKET_HITS 32u          /* hits a fill thread stages per stage */
 #define PSG_BUCKET_STAGE (PSG_BUCKET_THREADS * PSG_BUCKET_HITS)
 #define PSG_MAX_WINDOW_SEGS PSG_BUCKET_THREADS   /* segments per bucket window */
 #define PSG_ENTRY_SEG_SHIFT 21u      /* staged entry = window segment << 21 | bit index */

@@ -484,7 +484,7 @@ __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
 //   sparse: the multiples of p in the window's value range directly, keeping
 //     those whose cofactor is coprime to 30 -- one small division instead of
 //     eight class starts.
-// A round stops after `limit` hits; the walk resumes there in the next round.
+// A stage stops after `limit` hits; the walk resumes there in the next stage.
 struct BucketWalk {
   uint32_t k;        // index of the prefetched prime
   uint32_t pn;       // the prefetched prime primes[k] (0: none left)
@@ -498,9 +498,9 @@ struct BucketWalk {
 };
 
 // emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ...; returns the
-// number of hits of this round.
+// number of hits of this stage.
 template <typename Emit>
-__device__ __forceinline__ uint32_t bucket_walk_round(const PsgBucketArgs& a, const PsgWindow& w,
+__device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, const PsgWindow& w,
                                                       BucketWalk& s, uint32_t stride,
                                                       const uint16_t* ctab, uint32_t limit,
                                                       Emit emit)
@@ -615,11 +615,11 @@ psg_sieve_count(PsgSieveArgs args)
 // Big-prime bucket tier (see PsgBucketArgs)
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kStageF = PSG_BUCKET_HITS;   // hits per thread per round
+constexpr uint32_t kStageF = PSG_BUCKET_HITS;   // hits per thread per stage
 
-// One pass per (window, chunk) workgroup, in rounds.  In a round every thread
+// One pass per (window, chunk) workgroup, in stages.  In a stage every thread
 // walks up to kStageF hits into its LDS stage slots while an LDS histogram
-// counts them per window segment.  Then each segment's run of the round takes
+// counts them per window segment.  Then each segment's run of the stage takes
 // its place in the segment's list with one global atomic (issued early, used
 // late), the hits are sorted by segment in place (every thread first holds
 // its hits in registers), and the sorted runs are written out contiguously:
@@ -656,7 +656,7 @@ psg_bucket_fill(PsgBucketArgs a)
   uint64_t appended = 0;
   __syncthreads();
   for (;;) {
-    const uint32_t n = bucket_walk_round(a, w, st, stride, ctab, kStageF,
+    const uint32_t n = bucket_walk_stage(a, w, st, stride, ctab, kStageF,
         [&](uint32_t i, uint32_t seg, uint32_t bitIdx) {
           stage[i * nthreads + tid] = (seg << PSG_ENTRY_SEG_SHIFT) | bitIdx;
           atomicAdd(&hist[seg], 1u);
