@@ -345,16 +345,20 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     // tiers 1 and 2 (256 < p <= S): lane = prime, 64 consecutive primes per
     // wave; each class of p is struck as four interleaved progressions with
     // stride 4p, whose word masks are loop-invariant.
+    // Batches get lighter as p grows, so they are dealt to the waves in
+    // snake order (w, 2*nwaves-1-w, 2*nwaves+w, ...) to even out the load.
     {
       const uint32_t nb = (t2 - t0 + 63u) >> 6;
-      uint32_t b = wave;
+      const auto batchOf = [&](uint32_t r) { return r * nwaves + ((r & 1u) ? nwaves - 1u - wave : wave); };
+      uint32_t r = 0;
+      uint32_t b = batchOf(0);
       uint32_t kN = t0 + (b << 6) + lane;
       uint32_t pN = (b < nb && kN < t2) ? primes[kN] : 0u;
       uint64_t MN = (b < nb && kN < t2) ? magics[kN] : 0ull;
-      for (; b < nb; b += nwaves) {
+      for (; b < nb; b = batchOf(r)) {
         const uint32_t p = pN;
         const uint64_t M = MN;
-        const uint32_t bn = b + nwaves;
+        const uint32_t bn = batchOf(++r);
         kN = t0 + (bn << 6) + lane;
         pN = (bn < nb && kN < t2) ? primes[kN] : 0u;
         MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
