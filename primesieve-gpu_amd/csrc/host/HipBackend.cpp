@@ -104,10 +104,10 @@ void HipBackend::init()
   segmentsPerLaunch_ = config_.segmentsPerLaunch ? config_.segmentsPerLaunch : 65536u;
   bucketFactor_ = (config_.bucketFactor == 0) ? 8u
                 : (config_.bucketFactor == 0xffffffffu) ? 0u : config_.bucketFactor;
-  // a slab spans ~128 MiB of sieve bytes, at most 2048 segments (the count
-  // and fill kernels keep one LDS counter per (segment, class): 64 KiB)
+  // a slab spans ~1 GiB of sieve bytes (fewer launch tails; its bucket
+  // entries, ~4 B per crossing of a big prime, fit easily in HBM)
   uint32_t slab = config_.slabSegments ? config_.slabSegments
-                                       : (uint32_t) ((128ull << 20) / sieveBytes_);
+                                       : (uint32_t) ((1ull << 30) / sieveBytes_);
   slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 8192));
   windowSegs_ = 64;
   bucketMargin_ = config_.bucketMargin ? config_.bucketMargin : 6u;

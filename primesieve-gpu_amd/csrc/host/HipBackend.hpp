@@ -64,7 +64,7 @@ struct ExecOptions
   uint32_t windowSegs = 64;       // segments per bucket window
   uint32_t primesPerChunk = 512;  // big primes per fill workgroup ...
   uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
-  double entryBudget = 5.0e8;     // expected bucket entries per slab
+  double entryBudget = 1.0e9;     // expected bucket entries per slab (4 GB)
 };
 
 class HipBackend : public primesieve::gpu::GpuBackend
