@@ -240,8 +240,11 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
   uint32_t shift = 0;
   while ((1u << shift) < S) shift++;
   const bool pow2 = (1u << shift) == S;
+  // bucket primes: p > bucketFactor * S, and always p > 2^16 (the fill
+  // kernel's remainders rely on it)
   bigBegin_ = (pow2 && bucketFactor_)
-            ? primesUpTo((uint64_t) S * bucketFactor_) : (uint32_t) hostPrimes_.size();
+            ? primesUpTo(std::max<uint64_t>((uint64_t) S * bucketFactor_, 1u << 16))
+            : (uint32_t) hostPrimes_.size();
   const uint32_t windowSegs = std::max<uint32_t>(1, std::min(opt.windowSegs, PSG_MAX_WINDOW_SEGS));
 
   // ---- runs -----------------------------------------------------------------
@@ -415,7 +418,6 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   PsgBucketArgs b;
   std::memset(&b, 0, sizeof b);
   b.primes = primes_.addr();
-  b.magics = magics_.addr();
   b.entries = entries_.addr();
   b.overflow = flags_.addr();
   b.total = flags_.addr() + 8;

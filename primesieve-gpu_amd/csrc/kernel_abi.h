@@ -104,8 +104,7 @@ typedef struct PsgWindow {
 /* Arguments of psg_bucket_fill (one workgroup per (window, chunk)). */
 typedef struct PsgBucketArgs {
   uint64_t windows;     /* const PsgWindow*, sorted by wgBase */
-  uint64_t primes;      /* const uint32_t* */
-  uint64_t magics;      /* const uint64_t* */
+  uint64_t primes;      /* const uint32_t*; every bucket prime is >= 2^16 */
   uint64_t listOff;     /* const uint32_t[nSegs + 1] */
   uint64_t listCount;   /* uint32_t[nSegs], zero at launch */
   uint64_t entries;     /* uint32_t* */

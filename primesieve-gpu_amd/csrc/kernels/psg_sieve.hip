@@ -64,6 +64,37 @@ __device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
   return (x << r) | (x >> (32u - r));
 }
 
+// B mod p for p >= 2^16 and B < 2^60 without a division magic.  Bf is the
+// double nearest B (off by at most 2^6).  The hardware reciprocal is only an
+// estimate, so two Newton steps r(2 - p r) (each squares the relative error)
+// bring it to ~2^-51 from any estimate better than ~2^-13; then
+// q = floor(Bf * r) is within one of floor(B / p), since
+// B / p < 2^44 keeps the product's error far below 1.  The remainder is
+// computed exactly in 64-bit integers and corrected by up to two p each way.
+__device__ __forceinline__ double to_double(uint64_t B)
+{
+  return fma((double) (uint32_t) (B >> 32), 0x1p32, (double) (uint32_t) B);   // one rounding
+}
+
+__device__ __forceinline__ uint32_t mod_big(uint64_t B, double Bf, uint32_t p)
+{
+  const double pd = (double) p;
+  const double r0 = __builtin_amdgcn_rcp(pd);
+  const double r1 = fma(r0, fma(-pd, r0, 1.0), r0);
+  const double r2 = fma(r1, fma(-pd, r1, 1.0), r1);
+  const double qf = floor(Bf * r2);
+  const double qh = floor(qf * 0x1p-32);
+  const uint32_t qhi = (uint32_t) qh;
+  const uint32_t qlo = (uint32_t) (qf - qh * 0x1p32);      // exact: qf < 2^53
+  const uint64_t qp = (uint64_t) qlo * p + (((uint64_t) qhi * p) << 32);
+  int64_t r = (int64_t) (B - qp);
+  if (r < 0) r += p;
+  if (r < 0) r += p;
+  if (r >= (int64_t) p) r -= p;
+  if (r >= (int64_t) p) r -= p;
+  return (uint32_t) r;
+}
+
 // Start (byte index relative to segment base B) of residue class j of prime
 // p, given R = B mod p.  ctab[(p % 30) * 8 + j] packs q_j (the cofactor
 // residue mod 30 of class j) in bits 0..7 and e_j + 1 in bits 8..15, where
@@ -492,7 +523,6 @@ __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
 struct BucketWalk {
   uint32_t k;        // index of the prefetched prime
   uint32_t pn;       // the prefetched prime primes[k] (0: none left)
-  uint64_t Mn;       // its division magic
   uint32_t p;        // prime being walked (0: none)
   uint32_t R;        // window base mod p
   uint32_t j;        // dense: current class; 8 marks a sparse prime
@@ -510,9 +540,9 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
                                                       Emit emit)
 {
   const auto* __restrict__ primes = gptr<uint32_t>(a.primes);
-  const auto* __restrict__ magics = gptr<uint64_t>(a.magics);
   const uint32_t segMask = (1u << a.segShift) - 1u;
   const uint32_t bytes = w.segs << a.segShift;
+  const double baseF = to_double(w.base);
   const uint64_t V = 30ull * bytes;               // values covered by the window
   const uint64_t v0 = 30ull * w.base + 7u;        // its first value
   uint32_t n = 0;
@@ -521,10 +551,9 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
       if (s.pn == 0u) break;
       // take the prefetched prime and fetch the next one while this one is walked
       const uint32_t p = s.pn;
-      const uint32_t R = mod_magic(w.base, p, s.Mn);
+      const uint32_t R = mod_big(w.base, baseF, p);    // bucket primes are > 2^16
       s.k += stride;
       s.pn = (s.k < w.bigEnd) ? primes[s.k] : 0u;
-      s.Mn = (s.k < w.bigEnd) ? magics[s.k] : 0ull;
       s.p = p;
       s.R = R;
       if ((uint64_t) p * 8u > V) {
@@ -654,7 +683,6 @@ psg_bucket_fill(PsgBucketArgs a)
   BucketWalk st;
   st.k = a.bigBegin + c * nthreads + tid;
   st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
-  st.Mn = (st.k < w.bigEnd) ? gptr<uint64_t>(a.magics)[st.k] : 0ull;
   st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0ull;
   const uint32_t stride = nthreads * w.nChunks;
   uint64_t appended = 0;
