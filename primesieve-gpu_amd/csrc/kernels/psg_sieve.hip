@@ -76,13 +76,18 @@ __device__ __forceinline__ double to_double(uint64_t B)
   return fma((double) (uint32_t) (B >> 32), 0x1p32, (double) (uint32_t) B);   // one rounding
 }
 
-__device__ __forceinline__ uint32_t mod_big(uint64_t B, double Bf, uint32_t p)
+__device__ __forceinline__ double recip(uint32_t p)
 {
   const double pd = (double) p;
   const double r0 = __builtin_amdgcn_rcp(pd);
   const double r1 = fma(r0, fma(-pd, r0, 1.0), r0);
-  const double r2 = fma(r1, fma(-pd, r1, 1.0), r1);
-  const double qf = floor(Bf * r2);
+  return fma(r1, fma(-pd, r1, 1.0), r1);
+}
+
+// rp = recip(p)
+__device__ __forceinline__ uint32_t mod_big(uint64_t B, double Bf, uint32_t p, double rp)
+{
+  const double qf = floor(Bf * rp);
   const double qh = floor(qf * 0x1p-32);
   const uint32_t qhi = (uint32_t) qh;
   const uint32_t qlo = (uint32_t) (qf - qh * 0x1p32);      // exact: qf < 2^53
@@ -528,7 +533,8 @@ struct BucketWalk {
   uint32_t j;        // dense: current class; 8 marks a sparse prime
   uint32_t pos;      // dense: next byte of class j, relative to the window
   uint32_t qm;       // sparse: cofactor of the next multiple, mod 30
-  uint64_t off;      // sparse: next multiple minus the window's first value
+  uint32_t off;      // sparse: next multiple minus the window's first value (< 2^31)
+  uint32_t self;     // sparse: offset of p itself when it lies in the window, else ~0
 };
 
 // emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ...; returns the
@@ -543,7 +549,7 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
   const uint32_t segMask = (1u << a.segShift) - 1u;
   const uint32_t bytes = w.segs << a.segShift;
   const double baseF = to_double(w.base);
-  const uint64_t V = 30ull * bytes;               // values covered by the window
+  const uint32_t V = 30u * bytes;                 // values covered by the window (< 2^31)
   const uint64_t v0 = 30ull * w.base + 7u;        // its first value
   uint32_t n = 0;
   while (n < limit) {
@@ -551,7 +557,8 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
       if (s.pn == 0u) break;
       // take the prefetched prime and fetch the next one while this one is walked
       const uint32_t p = s.pn;
-      const uint32_t R = mod_big(w.base, baseF, p);    // bucket primes are > 2^16
+      const double rp = recip(p);
+      const uint32_t R = mod_big(w.base, baseF, p, rp);    // bucket primes are > 2^16
       s.k += stride;
       s.pn = (s.k < w.bigEnd) ? primes[s.k] : 0u;
       s.p = p;
@@ -560,12 +567,13 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
         // v0 = 30*Q*p + x with x = 30R + 7 < 31p; the first multiple of p at
         // or after v0 is v0 + off with cofactor 30*Q + ceil(x / p)
         const uint64_t x = 30ull * R + 7u;
-        uint32_t t = (uint32_t) ((float) x / (float) p);
-        while ((uint64_t) t * p > x) t--;
-        while ((uint64_t) (t + 1u) * p <= x) t++;
-        const uint32_t u = (uint32_t) (x - (uint64_t) t * p);
-        s.off = u ? (uint64_t) (p - u) : 0ull;
+        uint32_t t = (uint32_t) floor((double) x * rp);       // within one of x / p
+        int64_t u = (int64_t) (x - (uint64_t) t * p);
+        if (u < 0) { t--; u += p; }
+        if (u >= (int64_t) p) { t++; u -= p; }
+        s.off = u ? p - (uint32_t) u : 0u;
         s.qm = (t + (u ? 1u : 0u)) % 30u;
+        s.self = (p >= v0 && p - v0 < V) ? (uint32_t) (p - v0) : 0xffffffffu;
         s.j = 8u;
       } else {
         s.j = 0u;
@@ -573,13 +581,14 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
       }
     }
     if (s.j == 8u) {
-      for (; n < limit && s.off < V; s.off += s.p) {
-        if (((kCoprime30 >> s.qm) & 1u) && v0 + s.off != (uint64_t) s.p) {
-          const uint32_t byte = (uint32_t) (s.off / 30u);
-          const uint32_t bit = bit_of_residue((uint32_t) (s.off - 30ull * byte));
+      while (n < limit && s.off < V) {
+        if (((kCoprime30 >> s.qm) & 1u) && s.off != s.self) {
+          const uint32_t byte = s.off / 30u;
+          const uint32_t bit = bit_of_residue(s.off - 30u * byte);
           emit(n++, byte >> a.segShift, ((byte & segMask) << 3) | bit);
         }
         s.qm = (s.qm == 29u) ? 0u : s.qm + 1u;
+        s.off = (V - s.off > s.p) ? s.off + s.p : V;       // no 32-bit overflow
       }
       if (s.off >= V) s.p = 0u;
     } else {
@@ -683,7 +692,7 @@ psg_bucket_fill(PsgBucketArgs a)
   BucketWalk st;
   st.k = a.bigBegin + c * nthreads + tid;
   st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
-  st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0ull;
+  st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
   const uint32_t stride = nthreads * w.nChunks;
   uint64_t appended = 0;
   __syncthreads();
