@@ -16,8 +16,9 @@ Prints ONE JSON line:
                 primes >= 53 -- 7..47 are pre-sieve tiles: one 4-byte read
                 + one 4-byte write each; W =
                 32-bit words of the bit array: one fill write + one count
-                read); peak = the same 8-bytes-per-ds_and_b32 figure measured
-                by the conflict-free LDS microbenchmark on the same device;
+                read); peak = the same 8-bytes-per-ds_or_b32 figure (the
+                sieve's strike) measured by the conflict-free LDS
+                microbenchmark on the same device;
                 traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE /
                 WRITE_SIZE summary in profiles/ when present
   cpu_baseline  the reference CPU sieve (oracle/_ref, built from the
@@ -239,13 +240,13 @@ def main():
     lds_bytes_launch = (8 * work.crossings + 8 * work.words32) / max(1, launches)
     achieved = lds_bytes_launch / (avg_launch_ms / 1e3) / 1e9
     lds = m.lds_roofline()
-    peak = lds["and_conflict_free_gbs"]
+    peak = lds["or_conflict_free_gbs"]          # the sieve strikes with ds_or_b32
     roofline = {"bound": "lds", "achieved": achieved, "peak": peak, "unit": "GB/s",
                 "frac": achieved / peak, "traffic": profiled_traffic(),
                 "kernel": "psg_sieve_count", "avg_launch_ms": avg_launch_ms,
                 "launches_per_step": launches, "lds_bytes_per_launch": lds_bytes_launch,
                 "crossings": work.crossings, "words32": work.words32,
-                "peak_source": "measured ds_and_b32 microbenchmark (psg_lds_and, conflict-free)",
+                "peak_source": "measured ds_or_b32 microbenchmark (psg_lds_and mode 2, conflict-free)",
                 "measured": lds}
     out = {
         "metric": "integers sieved/s on 1 MI355X, exact counts",

@@ -141,7 +141,8 @@ typedef struct psg_work {
 int psg_algorithmic_work(uint64_t start, uint64_t stop, uint32_t seg_bytes, psg_work* out);
 
 /* Measured ceilings of the active device (microbenchmark kernels):
- * ds_and_b32 throughput on LDS with conflict-free and with random word
+ * ds_and_b32 (and ds_or_b32, the sieve's strike) throughput on LDS with
+ * conflict-free and with random word
  * addresses (GB/s counted as 8 bytes -- one 4-byte read plus one 4-byte
  * write -- per atomic, the convention of the sieve's LDS roofline), LDS
  * ds_read_b64 streaming, and HBM copy (read + write bytes).  0 on success. */
@@ -154,6 +155,8 @@ typedef struct psg_roofline {
   double hbm_copy_gbs;
   uint32_t compute_units;
   uint32_t reserved;
+  double or_conflict_free_gbs;     /* the same for ds_or_b32, the sieve's strike */
+  double or_random_gbs;
 } psg_roofline;
 int psg_measure_roofline(psg_roofline* out);
 

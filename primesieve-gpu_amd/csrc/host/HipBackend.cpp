@@ -109,7 +109,7 @@ void HipBackend::init()
   uint32_t slab = config_.slabSegments ? config_.slabSegments
                                        : (uint32_t) ((1ull << 30) / sieveBytes_);
   slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 8192));
-  windowSegs_ = 64;
+  windowSegs_ = 128;
   bucketMargin_ = config_.bucketMargin ? config_.bucketMargin : 6u;
   firstCrossPrime_ = config_.preSieve ? PSG_FIRST_CROSS_PRIME : 7u;
 
@@ -141,7 +141,7 @@ void HipBackend::init()
         throw primesieve_error("pre-sieve tile layout mismatch");
       std::vector<uint32_t> tile((size_t) 4 * Q, 0);
       for (uint32_t i = 0; i < periods[t]; i++)
-        tile[(size_t) (i % 4) * Q + i / 4] = w[i];
+        tile[(size_t) (i % 4) * Q + i / 4] = ~w[i];      // the kernel marks composites
       all.insert(all.end(), tile.begin(), tile.end());
     }
     if (all.size() != PSG_TILE_WORDS)

@@ -133,7 +133,7 @@ private:
   uint32_t bucketFactor_ = 8;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
   uint32_t slabSegments_ = 0;     // segments per slab of a single interval
   uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
-  uint32_t windowSegs_ = 64;      // segments per bucket window of a single interval
+  uint32_t windowSegs_ = 128;     // segments per bucket window of a single interval
   uint32_t persistentWGs_ = 0;    // resident workgroups of the sieve kernel
   uint32_t bucketMargin_ = 6;     // bucket list room: expected + margin * sqrt(expected) + 64
 

@@ -97,6 +97,12 @@ extern "C" int psg_measure_roofline(psg_roofline* out)
     t = timeLaunch(m, fAnd, grid, block, ldsBytes, pAnd);
     out->and_random_gops = atoms / t / 1e9;
     out->and_random_gbs = 8.0 * atoms / t / 1e9;
+    mode = 2;
+    t = timeLaunch(m, fAnd, grid, block, ldsBytes, pAnd);
+    out->or_conflict_free_gbs = 8.0 * atoms / t / 1e9;
+    mode = 3;
+    t = timeLaunch(m, fAnd, grid, block, ldsBytes, pAnd);
+    out->or_random_gbs = 8.0 * atoms / t / 1e9;
 
     void* pRead[] = { &words, &iters, &sinkp };
     t = timeLaunch(m, fRead, grid, block, ldsBytes, pRead);

@@ -24,7 +24,8 @@
 #define PSG_SIEVE_LDS_TAIL 512u  /* LDS after the segment: class table, counters */
 /* Pre-sieve tiles: the bit pattern of each group of small primes repeats
  * every P = product-of-the-group bytes.  A tile holds the P sliding 32-bit
- * windows of its pattern (window i = pattern bytes i..i+3, wrapping) in
+ * windows of its pattern (window i = pattern bytes i..i+3, wrapping; a set
+ * bit marks a multiple of a tile prime, as in the kernel's sieve) in
  * residue-major order: window i at word (i % 4) * Q + i / 4, Q = ceil(P/4),
  * so the windows 4 apart that consecutive sieve words need are adjacent.
  * Tiles are stored one after another; tiles 1.. (PSG_SMALL_TILE_WORDS

@@ -30,8 +30,9 @@
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
 //              pass; wave sums use ds_swizzle butterflies.
 //
-// Strikes are ds_and_b32 atomics on LDS words: several lanes may hit the
-// same word at once.
+// In LDS a set bit marks a composite (the counts read the complement), so a
+// strike is a ds_or_b32 atomic with a one-bit mask: several lanes may hit
+// the same word at once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -145,18 +146,18 @@ __device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint3
     uint32_t i = (cc >= R) ? cc - R : cc - R + p;
     if (kNear && q == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
-    const uint32_t m0 = ~(1u << (((i & 3u) << 3) + c));
+    const uint32_t m0 = 1u << (((i & 3u) << 3) + c);
     const uint32_t m1 = rotl32(m0, rot), m2 = rotl32(m1, rot), m3 = rotl32(m2, rot);
     uint32_t w0 = i & ~3u, w1 = (i + p) & ~3u, w2 = (i + 2u * p) & ~3u, w3 = (i + 3u * p) & ~3u;
     for (; w3 < S; w0 += step, w1 += step, w2 += step, w3 += step) {
-      atomicAnd(&sieve[w0 >> 2], m0);
-      atomicAnd(&sieve[w1 >> 2], m1);
-      atomicAnd(&sieve[w2 >> 2], m2);
-      atomicAnd(&sieve[w3 >> 2], m3);
+      atomicOr(&sieve[w0 >> 2], m0);
+      atomicOr(&sieve[w1 >> 2], m1);
+      atomicOr(&sieve[w2 >> 2], m2);
+      atomicOr(&sieve[w3 >> 2], m3);
     }
-    if (w0 < S) atomicAnd(&sieve[w0 >> 2], m0);
-    if (w1 < S) atomicAnd(&sieve[w1 >> 2], m1);
-    if (w2 < S) atomicAnd(&sieve[w2 >> 2], m2);
+    if (w0 < S) atomicOr(&sieve[w0 >> 2], m0);
+    if (w1 < S) atomicOr(&sieve[w1 >> 2], m1);
+    if (w2 < S) atomicOr(&sieve[w2 >> 2], m2);
   }
 }
 
@@ -178,14 +179,14 @@ __device__ __forceinline__ void strike_big(uint32_t* sieve, uint32_t p, uint32_t
     uint32_t i = (cc >= R) ? cc - R : cc - R + p;
     if (kNear && q == 1u && B + i < (uint64_t) p)
       i = 0xffffffffu;                  // the prime itself; its next hit is beyond S
-    if (i < S) atomicAnd(&sieve[i >> 2], ~(1u << (((i & 3u) << 3) + c)));
+    if (i < S) atomicOr(&sieve[i >> 2], 1u << (((i & 3u) << 3) + c));
   }
 }
 
 // Clear bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
 __device__ __forceinline__ void strike_bit(uint32_t* sieve, uint32_t v)
 {
-  atomicAnd(&sieve[v >> 5], ~(1u << (v & 31u)));
+  atomicOr(&sieve[v >> 5], 1u << (v & 31u));
 }
 
 // Strike one class progression from byte `pos` with word stride `step`
@@ -194,16 +195,16 @@ __device__ __forceinline__ void strike_bit(uint32_t* sieve, uint32_t v)
 __device__ __forceinline__ void strike_run(uint32_t* sieve, uint32_t pos, uint32_t step,
                                            uint32_t hi, uint32_t bit)
 {
-  const uint32_t mask = ~(1u << (((pos & 3u) << 3) + bit));
+  const uint32_t mask = 1u << (((pos & 3u) << 3) + bit);
   uint32_t addr = pos & ~3u;
   for (; addr + 3u * step < hi; addr += 4u * step) {
-    atomicAnd(&sieve[addr >> 2], mask);
-    atomicAnd(&sieve[(addr + step) >> 2], mask);
-    atomicAnd(&sieve[(addr + 2u * step) >> 2], mask);
-    atomicAnd(&sieve[(addr + 3u * step) >> 2], mask);
+    atomicOr(&sieve[addr >> 2], mask);
+    atomicOr(&sieve[(addr + step) >> 2], mask);
+    atomicOr(&sieve[(addr + 2u * step) >> 2], mask);
+    atomicOr(&sieve[(addr + 3u * step) >> 2], mask);
   }
   for (; addr < hi; addr += step)
-    atomicAnd(&sieve[addr >> 2], mask);
+    atomicOr(&sieve[addr >> 2], mask);
 }
 
 // Sum over the 64 lanes of a wave: xor butterflies inside each 32-lane half
@@ -260,8 +261,9 @@ __device__ void init_ctab(uint16_t* ctab)
   }
 }
 
-// Fill the segment from the pre-sieve tiles: word w is the AND over the
-// tiles of window (B + 4w) mod P.  Tile 0 comes from global memory, the small
+// Fill the segment from the pre-sieve tiles (stored complemented: a set bit
+// marks a multiple of a tile prime): word w is the OR over the tiles of
+// window (B + 4w) mod P.  Tile 0 comes from global memory, the small
 // tiles from their LDS copy (kLds) or global memory.  Eight words per thread
 // per trip, so that all their loads are in flight before the first store.
 template <bool kLds>
@@ -280,11 +282,11 @@ __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint
     stp[t] = (4u * nthreads) % kPeriod[t];
   }
   const auto word = [&](uint32_t w) {
-    uint32_t v = 0xffffffffu;
+    uint32_t v = 0u;
 #pragma unroll
     for (int t = 0; t < PSG_NTILES; t++) {
       const uint32_t pos = (idx[t] & 3u) * kRows[t] + (idx[t] >> 2);
-      v &= (t == 0 || !kLds) ? tiles[kOffset[t] + pos] : small[kOffset[t] - kOffset[1] + pos];
+      v |= (t == 0 || !kLds) ? tiles[kOffset[t] + pos] : small[kOffset[t] - kOffset[1] + pos];
       idx[t] += stp[t];
       idx[t] = min(idx[t], idx[t] - kPeriod[t]);     // below 0 wraps to a large value
     }
@@ -323,16 +325,16 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   if (!tiles) {
     // pre-sieving disabled: every prime >= 7 is struck in step 2
     for (uint32_t w = tid; w < words; w += nthreads)
-      sieve[w] = 0xffffffffu;
+      sieve[w] = 0u;
   } else {
     if (args.tilesInLds) fill_tiles<true>(sieve, words, B, tiles, small);
     else fill_tiles<false>(sieve, words, B, tiles, small);
-    // the tiles also cleared the tile primes 7..47 themselves: all of global
+    // the tiles also marked the tile primes 7..47 themselves: all of global
     // byte 0 (7..31) and bits 0-3 of global byte 1 (37..47).  A segment
     // that starts at byte 0 or 1 has them in its word 0, which thread 0
     // wrote above and restores without a barrier.
     if (B < 2u && tid == 0)
-      sieve[0] |= (B == 0) ? 0x0fffu : 0x000fu;
+      sieve[0] &= (B == 0) ? ~0x0fffu : ~0x000fu;
   }
   __syncthreads();
 
@@ -461,7 +463,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     const uint64_t* s64 = (const uint64_t*) sieve;
     const uint32_t words64 = S >> 3;
     for (uint32_t w = tid; w < words64; w += nthreads) {
-      uint64_t x = s64[w];
+      uint64_t x = ~s64[w];
       const uint64_t gw = B + 8ull * w;
       if (gw <= run.gLo || gw + 7 >= run.gHi)
         x = mask_edges(x, gw, run);

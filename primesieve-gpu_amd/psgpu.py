@@ -64,6 +64,8 @@ class Roofline(ctypes.Structure):
         ("hbm_copy_gbs", ctypes.c_double),
         ("compute_units", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("or_conflict_free_gbs", ctypes.c_double),
+        ("or_random_gbs", ctypes.c_double),
     ]
 
 
