@@ -136,7 +136,7 @@ __device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint3
   const uint32_t a = p / 30u;
   const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
   const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
-  const uint32_t step = 4u * p;
+  const uint32_t* end = sieve + (S >> 2);
   const uint32_t rot = (p & 3u) << 3;            // 8 or 24: p is odd
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
@@ -148,16 +148,20 @@ __device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint3
       i += p;                                   // skip the prime itself
     const uint32_t m0 = 1u << (((i & 3u) << 3) + c);
     const uint32_t m1 = rotl32(m0, rot), m2 = rotl32(m1, rot), m3 = rotl32(m2, rot);
-    uint32_t w0 = i & ~3u, w1 = (i + p) & ~3u, w2 = (i + 2u * p) & ~3u, w3 = (i + 3u * p) & ~3u;
-    for (; w3 < S; w0 += step, w1 += step, w2 += step, w3 += step) {
-      atomicOr(&sieve[w0 >> 2], m0);
-      atomicOr(&sieve[w1 >> 2], m1);
-      atomicOr(&sieve[w2 >> 2], m2);
-      atomicOr(&sieve[w3 >> 2], m3);
+    // word pointers (4p bytes = p words per step); the LDS base is folded in once
+    uint32_t* q0 = sieve + (i >> 2);
+    uint32_t* q1 = sieve + ((i + p) >> 2);
+    uint32_t* q2 = sieve + ((i + 2u * p) >> 2);
+    uint32_t* q3 = sieve + ((i + 3u * p) >> 2);
+    for (; q3 < end; q0 += p, q1 += p, q2 += p, q3 += p) {
+      atomicOr(q0, m0);
+      atomicOr(q1, m1);
+      atomicOr(q2, m2);
+      atomicOr(q3, m3);
     }
-    if (w0 < S) atomicOr(&sieve[w0 >> 2], m0);
-    if (w1 < S) atomicOr(&sieve[w1 >> 2], m1);
-    if (w2 < S) atomicOr(&sieve[w2 >> 2], m2);
+    if (q0 < end) atomicOr(q0, m0);
+    if (q1 < end) atomicOr(q1, m1);
+    if (q2 < end) atomicOr(q2, m2);
   }
 }
 
@@ -196,15 +200,18 @@ __device__ __forceinline__ void strike_run(uint32_t* sieve, uint32_t pos, uint32
                                            uint32_t hi, uint32_t bit)
 {
   const uint32_t mask = 1u << (((pos & 3u) << 3) + bit);
-  uint32_t addr = pos & ~3u;
-  for (; addr + 3u * step < hi; addr += 4u * step) {
-    atomicOr(&sieve[addr >> 2], mask);
-    atomicOr(&sieve[(addr + step) >> 2], mask);
-    atomicOr(&sieve[(addr + 2u * step) >> 2], mask);
-    atomicOr(&sieve[(addr + 3u * step) >> 2], mask);
+  // word pointers: the LDS base is folded in once, outside the loops
+  uint32_t* q = sieve + (pos >> 2);
+  const uint32_t* end = sieve + (hi >> 2);
+  const uint32_t ws = step >> 2;
+  for (; q + 3u * ws < end; q += 4u * ws) {
+    atomicOr(q, mask);
+    atomicOr(q + ws, mask);
+    atomicOr(q + 2u * ws, mask);
+    atomicOr(q + 3u * ws, mask);
   }
-  for (; addr < hi; addr += step)
-    atomicOr(&sieve[addr >> 2], mask);
+  for (; q < end; q += ws)
+    atomicOr(q, mask);
 }
 
 // Sum over the 64 lanes of a wave: xor butterflies inside each 32-lane half
