@@ -165,3 +165,18 @@ def test_bucket_list_room_retry(gpu, oracle):
         assert st.bucket_entries == entries
     finally:
         gpu.set_tuning()
+
+
+def test_measured_ceilings(gpu):
+    """The roofline microbenchmarks the benchmark divides by: every figure is
+    positive, the conflict-free LDS atomics are at least as fast as random
+    ones, and the HBM copy is in the range of an MI355X stream."""
+    r = gpu.lds_roofline()
+    _say("ceilings", r)
+    for k in ("and_conflict_free_gbs", "and_random_gbs", "or_conflict_free_gbs", "or_random_gbs",
+              "lds_read_gbs", "hbm_copy_gbs"):
+        assert r[k] > 0, k
+    assert r["or_conflict_free_gbs"] >= 0.9 * r["or_random_gbs"]
+    assert r["and_conflict_free_gbs"] >= 0.9 * r["and_random_gbs"]
+    assert 500 < r["hbm_copy_gbs"] < 10000
+    assert r["compute_units"] >= 1
