@@ -562,8 +562,13 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
   const uint64_t v0 = 30ull * w.base + 7u;        // its first value
   uint32_t n = 0;
   while (n < limit) {
-    if (s.p == 0u) {
-      if (s.pn == 0u) break;
+    // The lanes of a wave take their next primes together, once every active
+    // lane has finished its current one: the set-up (remainder, first
+    // multiple or class start) then runs once per prime for the whole wave
+    // instead of interleaving with the hit loops of other lanes.
+    if (__all(s.p == 0u)) {
+      if (__all(s.pn == 0u)) break;
+      if (s.pn == 0u) continue;               // this lane is done; others take primes
       // take the prefetched prime and fetch the next one while this one is walked
       const uint32_t p = s.pn;
       const double rp = recip(p);
@@ -589,6 +594,7 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
         s.pos = class_start(p, R, w.base, 0u, ctab);
       }
     }
+    if (s.p == 0u) continue;                  // idle until the wave's next primes
     if (s.j == 8u) {
       while (n < limit && s.off < V) {
         if (((kCoprime30 >> s.qm) & 1u) && s.off != s.self) {
