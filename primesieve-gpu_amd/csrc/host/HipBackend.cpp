@@ -121,6 +121,14 @@ void HipBackend::init()
   for (const auto& f : fns)
     hipCheck(api_, api_->ModuleGetFunction(f.f, module_, f.name),
              (std::string("hipModuleGetFunction(") + f.name + ")").c_str());
+  // the segment kernel addresses its sieve words by LDS offset, which holds
+  // only while its dynamic LDS starts at 0, i.e. it has no static LDS
+  int staticLds = -1;
+  hipCheck(api_, api_->FuncGetAttribute(&staticLds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fnSieve_),
+           "hipFuncGetAttribute(psg_sieve_count)");
+  if (staticLds != 0)
+    throw primesieve_error("psg_sieve_count has " + std::to_string(staticLds) +
+                                       " bytes of static LDS; its sieve must start at LDS address 0");
   hipCheck(api_, api_->StreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   hipCheck(api_, api_->EventCreate(&evStart_), "hipEventCreate");
   hipCheck(api_, api_->EventCreate(&evStop_), "hipEventCreate");

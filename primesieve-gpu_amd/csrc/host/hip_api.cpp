@@ -70,6 +70,7 @@ void load()
   ok &= resolve(lib, "hipModuleGetFunction", a.ModuleGetFunction, missing);
   ok &= resolve(lib, "hipModuleLaunchKernel", a.ModuleLaunchKernel, missing);
   ok &= resolve(lib, "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor", a.ModuleOccupancy, missing);
+  ok &= resolve(lib, "hipFuncGetAttribute", a.FuncGetAttribute, missing);
   ok &= resolve(lib, "hipMalloc", a.Malloc, missing);
   ok &= resolve(lib, "hipFree", a.Free, missing);
   ok &= resolve(lib, "hipHostMalloc", a.HostMalloc, missing);

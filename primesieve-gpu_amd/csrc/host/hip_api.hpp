@@ -38,6 +38,7 @@ struct HipApi
   decltype(&hipModuleGetFunction) ModuleGetFunction = nullptr;
   decltype(&hipModuleLaunchKernel) ModuleLaunchKernel = nullptr;
   decltype(&hipModuleOccupancyMaxActiveBlocksPerMultiprocessor) ModuleOccupancy = nullptr;
+  decltype(&hipFuncGetAttribute) FuncGetAttribute = nullptr;
   hipError_t (*Malloc)(void**, size_t) = nullptr;
   decltype(&hipFree) Free = nullptr;
   hipError_t (*HostMalloc)(void**, size_t, unsigned int) = nullptr;

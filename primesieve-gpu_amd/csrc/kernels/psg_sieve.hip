@@ -101,27 +101,63 @@ __device__ __forceinline__ uint32_t mod_big(uint64_t B, double Bf, uint32_t p, d
   return (uint32_t) r;
 }
 
+// Class table entry: ctab[(p % 30) * 8 + j] packs q_j (the cofactor residue
+// mod 30 of class j) in bits 0..23 and e_j + 1 in bits 24..31, where
+// c_j = (p/30)*q_j + e_j is the global byte of p*q_j.  A 24-bit multiply reads
+// only the low 24 bits, so c_j + 1 = mad24(p/30, entry, entry >> 24).
+//
+// (c - R) mod p for c, R < p: the unsigned difference wraps above 2^32 - p
+// when c < R, and then adding p brings it back below p.
+__device__ __forceinline__ uint32_t sub_mod(uint32_t c, uint32_t R, uint32_t p)
+{
+  const uint32_t u = c - R;
+  return min(u, u + p);
+}
+
 // Start (byte index relative to segment base B) of residue class j of prime
-// p, given R = B mod p.  ctab[(p % 30) * 8 + j] packs q_j (the cofactor
-// residue mod 30 of class j) in bits 0..7 and e_j + 1 in bits 8..15, where
-// c_j = (p/30)*q_j + e_j is the global byte of p*q_j.  The result is < p,
-// except that the prime itself (q_j == 1, first period) is skipped; that
-// can only happen when B < p, so callers that know B >= p pass kNear = false.
+// p, given R = B mod p.  The result is < p, except that the prime itself
+// (q_j == 1, first period) is skipped; that can only happen when B < p, so
+// callers that know B >= p pass kNear = false.
 template <bool kNear = true>
 __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t B,
-                                                uint32_t j, const uint16_t* ctab)
+                                                uint32_t j, const uint32_t* ctab)
 {
   const uint32_t a = p / 30u;
   const uint32_t b = p - a * 30u;
   const uint32_t ent = ctab[b * 8u + j];
   const uint32_t q = ent & 0xffu;
-  const uint32_t c = __umul24(a, q) + (ent >> 8) - 1u;     // a < 2^24 for every caller
-  uint32_t i = (c >= R) ? c - R : c - R + p;
+  uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R + 1u, p);   // a < 2^24 for every caller
   if (kNear && q == 1u && B + i < (uint64_t) p) {
     const uint64_t ii = (uint64_t) i + p;
     i = (ii > 0xffffffffull) ? 0xffffffffu : (uint32_t) ii;
   }
   return i;
+}
+
+// The segment kernel declares no static LDS, so its dynamic LDS -- the sieve
+// first -- starts at LDS address 0 (the host checks the kernel's static LDS
+// size when it loads the code object).  A sieve byte index rounded down to a
+// multiple of 4 is then the LDS address of its 32-bit word, and the strikes
+// use it as is: no base address is added per strike.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+__device__ __forceinline__ lds_u32* lds_word(uint32_t byte)
+{
+  return (lds_u32*) (uintptr_t) (byte & ~3u);
+}
+
+__device__ __forceinline__ void lds_or(lds_u32* w, uint32_t mask)
+{
+  __hip_atomic_fetch_or(w, mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The 8 class entries of p's residue b = p mod 30: two 16-byte LDS reads.
+struct ClassRow { uint32_t e[8]; };
+__device__ __forceinline__ ClassRow class_row(const uint32_t* ctab, uint32_t b)
+{
+  const uint4* row = (const uint4*) &ctab[b * 8u];
+  const uint4 r0 = row[0], r1 = row[1];
+  return ClassRow{{ r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w }};
 }
 
 // All hits of a prime 256 < p <= S in the segment at B (R = B mod p).  Each
@@ -130,88 +166,88 @@ __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t
 // each keeps one word mask: the masks of consecutive hits differ by a
 // rotation of 8 * (p mod 4) bits.
 template <bool kNear>
-__device__ __forceinline__ void strike_medium(uint32_t* sieve, uint32_t p, uint32_t R, uint64_t B,
-                                              uint32_t S, const uint16_t* ctab)
+__device__ __forceinline__ void strike_medium(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
+                                              const uint32_t* ctab)
 {
   const uint32_t a = p / 30u;
-  const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
-  const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
-  const uint32_t* end = sieve + (S >> 2);
+  const ClassRow row = class_row(ctab, p - a * 30u);
+  const lds_u32* end = lds_word(S);
   const uint32_t rot = (p & 3u) << 3;            // 8 or 24: p is odd
+  const uint32_t R1 = R + 1u;
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
-    const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
-    const uint32_t q = ent & 0xffu;
-    const uint32_t cc = __umul24(a, q) + (ent >> 8) - 1u;    // p <= S: a < 2^24
-    uint32_t i = (cc >= R) ? cc - R : cc - R + p;
-    if (kNear && q == 1u && B + i < (uint64_t) p)
+    const uint32_t ent = row.e[c];
+    uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);    // p <= S: a < 2^24
+    if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
     const uint32_t m0 = 1u << (((i & 3u) << 3) + c);
     const uint32_t m1 = rotl32(m0, rot), m2 = rotl32(m1, rot), m3 = rotl32(m2, rot);
-    // word pointers (4p bytes = p words per step); the LDS base is folded in once
-    uint32_t* q0 = sieve + (i >> 2);
-    uint32_t* q1 = sieve + ((i + p) >> 2);
-    uint32_t* q2 = sieve + ((i + 2u * p) >> 2);
-    uint32_t* q3 = sieve + ((i + 3u * p) >> 2);
+    // word pointers; 4p bytes = p words per step
+    lds_u32* q0 = lds_word(i);
+    lds_u32* q1 = lds_word(i + p);
+    lds_u32* q2 = lds_word(i + 2u * p);
+    lds_u32* q3 = lds_word(i + 3u * p);
     for (; q3 < end; q0 += p, q1 += p, q2 += p, q3 += p) {
-      atomicOr(q0, m0);
-      atomicOr(q1, m1);
-      atomicOr(q2, m2);
-      atomicOr(q3, m3);
+      lds_or(q0, m0);
+      lds_or(q1, m1);
+      lds_or(q2, m2);
+      lds_or(q3, m3);
     }
-    if (q0 < end) atomicOr(q0, m0);
-    if (q1 < end) atomicOr(q1, m1);
-    if (q2 < end) atomicOr(q2, m2);
+    if (q0 < end) lds_or(q0, m0);
+    if (q1 < end) lds_or(q1, m1);
+    if (q2 < end) lds_or(q2, m2);
   }
 }
 
 // The <= 8 hits of a prime S < p in the segment at B (R = B mod p): one per
-// residue class at most.  The class table row of p is one 16-byte LDS read.
-// kSmall: p < 30 * 2^24, so that a * q is a 24-bit multiply.
+// residue class at most.  kSmall: p < 30 * 2^24, so that (p/30) * q_j is a
+// 24-bit multiply (and p < 2^31, which sub_mod needs).
 template <bool kNear, bool kSmall>
-__device__ __forceinline__ void strike_big(uint32_t* sieve, uint32_t p, uint32_t R, uint64_t B,
-                                           uint32_t S, const uint16_t* ctab)
+__device__ __forceinline__ void strike_big(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
+                                           const uint32_t* ctab)
 {
   const uint32_t a = p / 30u;
-  const uint4 row = *(const uint4*) &ctab[(p - a * 30u) * 8u];
-  const uint32_t wr[4] = { row.x, row.y, row.z, row.w };
+  const ClassRow row = class_row(ctab, p - a * 30u);
+  const uint32_t R1 = R + 1u;
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
-    const uint32_t ent = (wr[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu;
-    const uint32_t q = ent & 0xffu;
-    const uint32_t cc = (kSmall ? __umul24(a, q) : a * q) + (ent >> 8) - 1u;
-    uint32_t i = (cc >= R) ? cc - R : cc - R + p;
-    if (kNear && q == 1u && B + i < (uint64_t) p)
+    const uint32_t ent = row.e[c];
+    uint32_t i;
+    if (kSmall) {
+      i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);
+    } else {
+      const uint32_t cc = a * (ent & 0xffu) + (ent >> 24) - 1u;
+      i = (cc >= R) ? cc - R : cc - R + p;
+    }
+    if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
       i = 0xffffffffu;                  // the prime itself; its next hit is beyond S
-    if (i < S) atomicOr(&sieve[i >> 2], 1u << (((i & 3u) << 3) + c));
+    if (i < S) lds_or(lds_word(i), 1u << (((i & 3u) << 3) + c));
   }
 }
 
-// Clear bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
-__device__ __forceinline__ void strike_bit(uint32_t* sieve, uint32_t v)
+// Strike bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
+__device__ __forceinline__ void strike_bit(uint32_t v)
 {
-  atomicOr(&sieve[v >> 5], 1u << (v & 31u));
+  lds_or(lds_word(v >> 3), 1u << (v & 31u));
 }
 
 // Strike one class progression from byte `pos` with word stride `step`
 // (a multiple of 4, so the word mask is loop-invariant) up to byte `hi`
 // (a multiple of 4); four atomics per loop trip.
-__device__ __forceinline__ void strike_run(uint32_t* sieve, uint32_t pos, uint32_t step,
-                                           uint32_t hi, uint32_t bit)
+__device__ __forceinline__ void strike_run(uint32_t pos, uint32_t step, uint32_t hi, uint32_t bit)
 {
   const uint32_t mask = 1u << (((pos & 3u) << 3) + bit);
-  // word pointers: the LDS base is folded in once, outside the loops
-  uint32_t* q = sieve + (pos >> 2);
-  const uint32_t* end = sieve + (hi >> 2);
+  lds_u32* q = lds_word(pos);
+  const lds_u32* end = lds_word(hi);
   const uint32_t ws = step >> 2;
   for (; q + 3u * ws < end; q += 4u * ws) {
-    atomicOr(q, mask);
-    atomicOr(q + ws, mask);
-    atomicOr(q + 2u * ws, mask);
-    atomicOr(q + 3u * ws, mask);
+    lds_or(q, mask);
+    lds_or(q + ws, mask);
+    lds_or(q + 2u * ws, mask);
+    lds_or(q + 3u * ws, mask);
   }
   for (; q < end; q += ws)
-    atomicOr(q, mask);
+    lds_or(q, mask);
 }
 
 // Sum over the 64 lanes of a wave: xor butterflies inside each 32-lane half
@@ -249,18 +285,18 @@ __device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRun& r)
 
 // Residue-class table, derived from first principles: for b = p mod 30 and
 // class j (offset o_j), q_j = o_j * b^-1 (mod 30) and e_j = (b*q_j - o_j)/30,
-// packed as q_j | (e_j + 1) << 8 (e_j is -1 only for o_j = 31, b = 1).
-__device__ void init_ctab(uint16_t* ctab)
+// packed as q_j | (e_j + 1) << 24 (e_j is -1 only for o_j = 31, b = 1).
+__device__ void init_ctab(uint32_t* ctab)
 {
   for (uint32_t t = threadIdx.x; t < 240; t += blockDim.x) {
     const uint32_t b = t >> 3, j = t & 7u;
     const uint32_t o = kOffsets[j];
-    uint16_t ent = 0;
+    uint32_t ent = 0;
     if ((b % 2u) && (b % 3u) && (b % 5u)) {
       for (uint32_t q = 1; q < 30; q++)
         if ((b * q) % 30u == o % 30u) {
           const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
-          ent = (uint16_t) (q | ((uint32_t) (e + 1) << 8));
+          ent = q | ((uint32_t) (e + 1) << 24);
           break;
         }
     }
@@ -314,7 +350,7 @@ __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint
 // sieve and the wave sums may be reused by the next segment.
 __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const PsgRun& run,
                                               uint32_t seg, uint32_t* sieve,
-                                              const uint16_t* ctab, uint32_t* wsum,
+                                              const uint32_t* ctab, uint32_t* wsum,
                                               const uint32_t* small)
 {
   const uint32_t tid = threadIdx.x;
@@ -382,7 +418,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         for (uint32_t q = 0; q < 8; q++) {
           const uint32_t ps = (uint32_t) __builtin_amdgcn_readlane((int) p, (int) (q * 8u));
           if (!ps) break;
-          strike_run(sieve, sliceLo + sq[q] + slot * ps, 8u * ps, sliceHi, j);
+          strike_run(sliceLo + sq[q] + slot * ps, 8u * ps, sliceHi, j);
         }
       }
     }
@@ -409,8 +445,8 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
-          if (nearSmall) strike_medium<true>(sieve, p, R, B, S, ctab);
-          else strike_medium<false>(sieve, p, R, B, S, ctab);
+          if (nearSmall) strike_medium<true>(p, R, B, S, ctab);
+          else strike_medium<false>(p, R, B, S, ctab);
         }
       }
     }
@@ -436,9 +472,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
         MN = (bn < nb && kN < t3) ? magics[kN] : 0ull;
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
-          if (near) strike_big<true, false>(sieve, p, R, B, S, ctab);
-          else if (small) strike_big<false, true>(sieve, p, R, B, S, ctab);
-          else strike_big<false, false>(sieve, p, R, B, S, ctab);
+          if (near) strike_big<true, false>(p, R, B, S, ctab);
+          else if (small) strike_big<false, true>(p, R, B, S, ctab);
+          else strike_big<false, false>(p, R, B, S, ctab);
         }
       }
     }
@@ -457,10 +493,10 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
       const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
       const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
-      strike_bit(sieve, v0); strike_bit(sieve, v1);
-      strike_bit(sieve, v2); strike_bit(sieve, v3);
+      strike_bit(v0); strike_bit(v1);
+      strike_bit(v2); strike_bit(v3);
     }
-    for (; e < e1; e += nthreads) strike_bit(sieve, ent[e]);
+    for (; e < e1; e += nthreads) strike_bit(ent[e]);
   }
   __syncthreads();
 
@@ -551,7 +587,7 @@ struct BucketWalk {
 template <typename Emit>
 __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, const PsgWindow& w,
                                                       BucketWalk& s, uint32_t stride,
-                                                      const uint16_t* ctab, uint32_t limit,
+                                                      const uint32_t* ctab, uint32_t limit,
                                                       Emit emit)
 {
   const auto* __restrict__ primes = gptr<uint32_t>(a.primes);
@@ -637,8 +673,8 @@ psg_sieve_count(PsgSieveArgs args)
   // class table and the block's small variables follow it
   extern __shared__ uint32_t lds[];
   uint32_t* sieve = lds;
-  uint16_t* ctab = (uint16_t*) (lds + (args.segBytes >> 2));   // 240 entries, 16-byte aligned
-  uint32_t* wsum = lds + (args.segBytes >> 2) + 120u;          // PSG_NCOUNTS
+  uint32_t* ctab = lds + (args.segBytes >> 2);                 // 240 entries, 16-byte aligned
+  uint32_t* wsum = ctab + 240u;                                 // PSG_NCOUNTS
   uint32_t& sSeg = wsum[PSG_NCOUNTS];
   uint32_t& sRun = wsum[PSG_NCOUNTS + 1];
   // the small pre-sieve tiles, copied once per workgroup when they fit
@@ -691,7 +727,7 @@ psg_bucket_fill(PsgBucketArgs a)
   uint32_t* hist = stage + PSG_BUCKET_STAGE;           // [W] counts, then scatter cursors
   uint32_t* place = hist + PSG_MAX_WINDOW_SEGS;        // [W] first sorted slot | fit << 16
   uint32_t* dst = place + PSG_MAX_WINDOW_SEGS;         // [W] entry index of the segment's run
-  __shared__ uint16_t ctab[240];
+  __shared__ __align__(16) uint32_t ctab[240];
   __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
 
   init_ctab(ctab);

@@ -29,6 +29,7 @@ hipError_t hipModuleUnload(void* m) { (void) m; return ok; }
 hipError_t hipModuleGetFunction(void** f, void* m, const char* n) { (void) m; (void) n; *f = (void*) 1; return ok; }
 hipError_t hipModuleLaunchKernel() { return errInvalid; }
 hipError_t hipModuleOccupancyMaxActiveBlocksPerMultiprocessor() { return errInvalid; }
+hipError_t hipFuncGetAttribute(int* v, int attr, void* f) { (void) attr; (void) f; *v = 0; return ok; }
 static int nolaunch(void) { return strcmp(mode(), "nolaunch") == 0; }
 hipError_t hipMalloc(void** p, size_t n) { if (!nolaunch()) { *p = NULL; return errOOM; } *p = malloc(n ? n : 1); return ok; }
 hipError_t hipFree(void* p) { if (nolaunch()) free(p); return ok; }
