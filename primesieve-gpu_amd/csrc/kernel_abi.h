@@ -87,7 +87,6 @@ typedef struct PsgSieveArgs {
 #define PSG_BUCKET_HITS 32u          /* hits a fill thread stages per stage */
 #define PSG_BUCKET_STAGE (PSG_BUCKET_THREADS * PSG_BUCKET_HITS)
 #define PSG_MAX_WINDOW_SEGS PSG_BUCKET_THREADS   /* segments per bucket window */
-#define PSG_ENTRY_SEG_SHIFT 21u      /* staged entry = window segment << 21 | bit index */
 
 /* One bucket window (32 bytes): consecutive segments of one run walked by
  * nChunks workgroups, each over an interleaved share of the big primes

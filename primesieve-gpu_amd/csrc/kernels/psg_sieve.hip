@@ -582,8 +582,9 @@ struct BucketWalk {
   uint32_t self;     // sparse: offset of p itself when it lies in the window, else ~0
 };
 
-// emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ...; returns the
-// number of hits of this stage.
+// emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ..., with byte
+// and bit within the window (so the window segment is the hit >> (segShift +
+// 3)); returns the number of hits of this stage.
 template <typename Emit>
 __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, const PsgWindow& w,
                                                       BucketWalk& s, uint32_t stride,
@@ -591,7 +592,6 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
                                                       Emit emit)
 {
   const auto* __restrict__ primes = gptr<uint32_t>(a.primes);
-  const uint32_t segMask = (1u << a.segShift) - 1u;
   const uint32_t bytes = w.segs << a.segShift;
   const double baseF = to_double(w.base);
   const uint32_t V = 30u * bytes;                 // values covered by the window (< 2^31)
@@ -636,15 +636,20 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
         if (((kCoprime30 >> s.qm) & 1u) && s.off != s.self) {
           const uint32_t byte = s.off / 30u;
           const uint32_t bit = bit_of_residue(s.off - 30u * byte);
-          emit(n++, byte >> a.segShift, ((byte & segMask) << 3) | bit);
+          emit(n++, byte >> a.segShift, (byte << 3) | bit);
         }
         s.qm = (s.qm == 29u) ? 0u : s.qm + 1u;
         s.off = (V - s.off > s.p) ? s.off + s.p : V;       // no 32-bit overflow
       }
       if (s.off >= V) s.p = 0u;
     } else {
-      for (; n < limit && s.pos < bytes; s.pos += s.p)
-        emit(n++, s.pos >> a.segShift, ((s.pos & segMask) << 3) | s.j);
+      if (s.pos < bytes) {
+        // one exit test per hit: stop at the window's end or when the stage is full
+        const uint32_t room = limit - n;
+        const uint32_t stop = ((uint64_t) room * s.p >= bytes - s.pos) ? bytes : s.pos + room * s.p;
+        for (; s.pos < stop; s.pos += s.p)
+          emit(n++, s.pos >> a.segShift, (s.pos << 3) | s.j);
+      }
       if (s.pos >= bytes) {
         if (++s.j == 8u) s.p = 0u;
         else s.pos = class_start(s.p, s.R, w.base, s.j, ctab);
@@ -725,8 +730,8 @@ psg_bucket_fill(PsgBucketArgs a)
   const uint32_t nthreads = PSG_BUCKET_THREADS;        // the launch's block size
   uint32_t* stage = lds;                               // [kStageF][nthreads], then sorted
   uint32_t* hist = stage + PSG_BUCKET_STAGE;           // [W] counts, then scatter cursors
-  uint32_t* place = hist + PSG_MAX_WINDOW_SEGS;        // [W] first sorted slot | fit << 16
-  uint32_t* dst = place + PSG_MAX_WINDOW_SEGS;         // [W] entry index of the segment's run
+  // [W] (first sorted slot | fit << 16, entry index of the segment's run)
+  uint2* place = (uint2*) (hist + PSG_MAX_WINDOW_SEGS);
   __shared__ __align__(16) uint32_t ctab[240];
   __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
 
@@ -740,6 +745,8 @@ psg_bucket_fill(PsgBucketArgs a)
   const uint32_t lane = tid & 63u, wave = tid >> 6, nwaves = nthreads >> 6;
   for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
 
+  const uint32_t eshift = a.segShift + 3u;             // window bit index -> window segment
+  const uint32_t segBits = (1u << eshift) - 1u;        // window bit index -> segment bit index
   BucketWalk st;
   st.k = a.bigBegin + c * nthreads + tid;
   st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
@@ -749,8 +756,8 @@ psg_bucket_fill(PsgBucketArgs a)
   __syncthreads();
   for (;;) {
     const uint32_t n = bucket_walk_stage(a, w, st, stride, ctab, kStageF,
-        [&](uint32_t i, uint32_t seg, uint32_t bitIdx) {
-          stage[i * nthreads + tid] = (seg << PSG_ENTRY_SEG_SHIFT) | bitIdx;
+        [&](uint32_t i, uint32_t seg, uint32_t hit) {
+          stage[i * nthreads + tid] = hit;
           atomicAdd(&hist[seg], 1u);
         });
     const int more = __syncthreads_or((st.p != 0u || st.pn != 0u) ? 1 : 0);
@@ -783,13 +790,12 @@ psg_bucket_fill(PsgBucketArgs a)
 
 #pragma unroll
     for (uint32_t u = 0; u < kStageF; u++)
-      if (u < n) stage[atomicAdd(&hist[e[u] >> PSG_ENTRY_SEG_SHIFT], 1u)] = e[u];
+      if (u < n) stage[atomicAdd(&hist[e[u] >> eshift], 1u)] = e[u];
     if (tid < W) {
       const uint32_t room = listOff[tid + 1u] - listOff[tid];
       if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
       const uint32_t fit = (at >= room) ? 0u : min(cnt, room - at);
-      place[tid] = f | (fit << 16);
-      dst[tid] = listOff[tid] + at;
+      place[tid] = make_uint2(f | (fit << 16), listOff[tid] + at);
     }
     __syncthreads();
 
@@ -797,10 +803,9 @@ psg_bucket_fill(PsgBucketArgs a)
     for (uint32_t u = 0; u < kStageF; u++) {
       const uint32_t i = u * nthreads + tid;
       const uint32_t v = stage[i];
-      const uint32_t seg = (v >> PSG_ENTRY_SEG_SHIFT) & (PSG_MAX_WINDOW_SEGS - 1u);
-      const uint32_t pl = place[seg];
-      const uint32_t r = i - (pl & 0xffffu);
-      if (i < total && r < (pl >> 16)) entries[dst[seg] + r] = v & ((1u << PSG_ENTRY_SEG_SHIFT) - 1u);
+      const uint2 pl = place[(v >> eshift) & (PSG_MAX_WINDOW_SEGS - 1u)];
+      const uint32_t r = i - (pl.x & 0xffffu);
+      if (i < total && r < (pl.x >> 16)) entries[pl.y + r] = v & segBits;
     }
     appended += total;
     for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
