@@ -419,7 +419,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   args.counts = counts_.addr();
   args.segBytes = S;
   args.tier0End = primesUpTo(std::min<uint32_t>(PSG_TIER0_MAX_PRIME, S / 8));
-  args.tier1End = primesUpTo(S / 8);
+  args.tier1End = primesUpTo(std::min<uint32_t>(PSG_TIER1_MAX_PRIME, S));
   args.tier2End = primesUpTo(S);
   args.tier3End = bigBegin_;
 

@@ -38,6 +38,7 @@
 #define PSG_SMALL_TILE_WORDS (PSG_TILE_WORDS - 17020u)
 #define PSG_FIRST_CROSS_PRIME 53u
 #define PSG_TIER0_MAX_PRIME 256u
+#define PSG_TIER1_MAX_PRIME 2048u
 
 /* One run (48 bytes). */
 typedef struct PsgRun {
@@ -69,7 +70,7 @@ typedef struct PsgSieveArgs {
   uint32_t nSegs;       /* segments in the slab */
   uint32_t segBytes;    /* multiple of 256, <= PSG_MAX_SEG_BYTES */
   uint32_t tier0End;    /* primes[0, tier0End): p <= 256, struck per wave slice */
-  uint32_t tier1End;    /* primes[tier0End, tier1End): p <= segBytes/8 */
+  uint32_t tier1End;    /* primes[tier0End, tier1End): p <= min(PSG_TIER1_MAX_PRIME, segBytes) */
   uint32_t tier2End;    /* primes[tier1End, tier2End): p <= segBytes */
   uint32_t tier3End;    /* primes[tier2End, tier3End): per-segment big primes;
                            larger primes come from the bucket lists */

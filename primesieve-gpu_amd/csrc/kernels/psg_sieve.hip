@@ -21,10 +21,15 @@
 //                tier 0 (p <= 256): each wave strikes all of them over its
 //                        own slice of the segment (even load); lane =
 //                        (class, hit), stride 8p, loop-invariant word mask;
-//                tiers 1-2 (256 < p <= S): lane = prime; each class as four
-//                        progressions of stride 4p with loop-invariant masks;
+//                tier 1 (256 < p <= 2048): lane = (prime, class), 8 primes
+//                        per work item; each class as four progressions of
+//                        stride 4p with loop-invariant masks;
+//                tier 2 (2048 < p <= S): lane = prime, 64 per item, the same
+//                        progressions for each of its classes;
 //                tier 3 (S < p <= bucket threshold): lane = prime, at most
 //                        one hit per class;
+//              the items of tiers 1-3 form one queue per segment that the
+//              waves take from, heaviest first;
 //              plus the bucket list of the segment, filled by
 //              psg_bucket_fill for larger primes (one LDS atomic per entry);
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
@@ -160,19 +165,41 @@ __device__ __forceinline__ ClassRow class_row(const uint32_t* ctab, uint32_t b)
   return ClassRow{{ r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w }};
 }
 
-// All hits of a prime 256 < p <= S in the segment at B (R = B mod p).  Each
-// residue class is an arithmetic progression i, i+p, i+2p, ... of bytes; it
-// is struck as four progressions with stride 4p (a multiple of 4 bytes), so
-// each keeps one word mask: the masks of consecutive hits differ by a
-// rotation of 8 * (p mod 4) bits.
+// All hits of residue class c (bit c) of a prime p <= S from byte i of the
+// segment on: the progression i, i+p, i+2p, ... is struck as four
+// progressions with stride 4p (a multiple of 4 bytes), so each keeps one
+// word mask: the masks of consecutive hits differ by a rotation of
+// 8 * (p mod 4) bits.
+__device__ __forceinline__ void strike_class(uint32_t i, uint32_t p, uint32_t c, uint32_t S)
+{
+  const lds_u32* end = lds_word(S);
+  const uint32_t rot = (p & 3u) << 3;            // 8 or 24: p is odd
+  const uint32_t m0 = 1u << (((i & 3u) << 3) + c);
+  const uint32_t m1 = rotl32(m0, rot), m2 = rotl32(m1, rot), m3 = rotl32(m2, rot);
+  // word pointers; 4p bytes = p words per step
+  lds_u32* q0 = lds_word(i);
+  lds_u32* q1 = lds_word(i + p);
+  lds_u32* q2 = lds_word(i + 2u * p);
+  lds_u32* q3 = lds_word(i + 3u * p);
+  for (; q3 < end; q0 += p, q1 += p, q2 += p, q3 += p) {
+    lds_or(q0, m0);
+    lds_or(q1, m1);
+    lds_or(q2, m2);
+    lds_or(q3, m3);
+  }
+  if (q0 < end) lds_or(q0, m0);
+  if (q1 < end) lds_or(q1, m1);
+  if (q2 < end) lds_or(q2, m2);
+}
+
+// All hits of a prime p <= S in the segment at B (R = B mod p): its 8
+// residue classes, each an arithmetic progression of bytes with stride p.
 template <bool kNear>
 __device__ __forceinline__ void strike_medium(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
                                               const uint32_t* ctab)
 {
   const uint32_t a = p / 30u;
   const ClassRow row = class_row(ctab, p - a * 30u);
-  const lds_u32* end = lds_word(S);
-  const uint32_t rot = (p & 3u) << 3;            // 8 or 24: p is odd
   const uint32_t R1 = R + 1u;
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
@@ -180,22 +207,33 @@ __device__ __forceinline__ void strike_medium(uint32_t p, uint32_t R, uint64_t B
     uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);    // p <= S: a < 2^24
     if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
-    const uint32_t m0 = 1u << (((i & 3u) << 3) + c);
-    const uint32_t m1 = rotl32(m0, rot), m2 = rotl32(m1, rot), m3 = rotl32(m2, rot);
-    // word pointers; 4p bytes = p words per step
-    lds_u32* q0 = lds_word(i);
-    lds_u32* q1 = lds_word(i + p);
-    lds_u32* q2 = lds_word(i + 2u * p);
-    lds_u32* q3 = lds_word(i + 3u * p);
-    for (; q3 < end; q0 += p, q1 += p, q2 += p, q3 += p) {
-      lds_or(q0, m0);
-      lds_or(q1, m1);
-      lds_or(q2, m2);
-      lds_or(q3, m3);
+    strike_class(i, p, c, S);
+  }
+}
+
+// strike_medium for S/4 < p <= S, where a class has at most nh <= 4 hits
+// (nh is uniform over the wave): the hits of a class one after another, the
+// word mask rotated by 8 * (p mod 4) bits from one hit to the next.
+template <bool kNear>
+__device__ __forceinline__ void strike_few(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
+                                           const uint32_t* ctab, uint32_t nh)
+{
+  const uint32_t a = p / 30u;
+  const ClassRow row = class_row(ctab, p - a * 30u);
+  const uint32_t R1 = R + 1u;
+  const uint32_t rot = (p & 3u) << 3;
+#pragma unroll
+  for (uint32_t c = 0; c < 8; c++) {
+    const uint32_t ent = row.e[c];
+    uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);
+    if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
+      i += p;                                   // skip the prime itself
+    uint32_t m = 1u << (((i & 3u) << 3) + c);
+    for (uint32_t k = 0; k < nh; k++) {
+      if (i < S) lds_or(lds_word(i), m);
+      i += p;
+      m = rotl32(m, rot);
     }
-    if (q0 < end) lds_or(q0, m0);
-    if (q1 < end) lds_or(q1, m1);
-    if (q2 < end) lds_or(q2, m2);
   }
 }
 
@@ -346,6 +384,28 @@ __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint
   for (; w < words; w += nthreads) sieve[w] = word(w);
 }
 
+// Prime index of lane `lane` in work item `it` of a segment's queue (see
+// sieve_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each (lane =
+// prime * 8 + class), [u1, u2) and [u2, u3) 64 primes of [t1, t2) and
+// [t2, t3) (lane = prime).  `it` is uniform.
+__device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint32_t t0, uint32_t t1,
+                                               uint32_t t2, uint32_t t3, uint32_t u1, uint32_t u2,
+                                               uint32_t u3)
+{
+  uint32_t k, end;
+  if (it < u1) {
+    k = t0 + (it << 3) + (lane >> 3);
+    end = t1;
+  } else if (it < u2) {
+    k = t1 + ((it - u1) << 6) + lane;
+    end = t2;
+  } else {
+    k = t2 + ((it - u2) << 6) + lane;
+    end = t3;
+  }
+  return (it < u3 && k < end) ? k : 0xffffffffu;
+}
+
 // One segment: fill, strike, count.  Ends with a barrier, so that the LDS
 // sieve and the wave sums may be reused by the next segment.
 __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const PsgRun& run,
@@ -363,6 +423,8 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   const auto* __restrict__ primes = gptr<uint32_t>(args.primes);
   const auto* __restrict__ magics = gptr<uint64_t>(args.magics);
   const auto* __restrict__ tiles = gptr<uint32_t>(args.tiles);
+  uint32_t* sWork = wsum + PSG_NCOUNTS + 2;        // work queue of step 2, reset below
+  if (tid == 0) *sWork = 0u;
 
   // ---- 1. fill from the pre-sieve tiles ----------------------------------
   if (!tiles) {
@@ -382,9 +444,6 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   __syncthreads();
 
   // ---- 2. strike the sieving primes --------------------------------------
-  // Work is assigned statically (wave w takes batches w, w + nwaves, ...)
-  // and the next batch's primes are loaded before the current batch is
-  // struck, so that global-load latency overlaps LDS work.
   {
     const uint32_t np = run.nprimes;
     const uint32_t t0 = min(args.tier0End, np);
@@ -423,65 +482,71 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
       }
     }
 
-    // tiers 1 and 2 (256 < p <= S): lane = prime, 64 consecutive primes per
-    // wave; each class of p is struck as four interleaved progressions with
-    // stride 4p, whose word masks are loop-invariant.
-    // Batches get lighter as p grows, so they are dealt to the waves in
-    // snake order (w, 2*nwaves-1-w, 2*nwaves+w, ...) to even out the load.
+    // tiers 1-3: a queue of work items shared by the waves of the workgroup
+    // (an LDS counter), heaviest first, so that no wave is left with a long
+    // tail of work while the others wait at the barrier:
+    //   tier 1 (256 < p <= 2048): 8 primes per item, lane = (prime, class),
+    //     each lane strikes one class over the whole segment (the primes of
+    //     an item are close, so its lanes finish together);
+    //   tier 2 (2048 < p <= S): 64 primes per item, lane = prime;
+    //   tier 3 (S < p <= bucket threshold): 64 primes per item, lane = prime,
+    //     <= 1 hit per class; the prime itself can only be in the segment
+    //     while B < p (uniform test against the largest tier-3 prime).
+    // The next item's number and primes are fetched while the current item
+    // is struck.
     {
-      const uint32_t nb = (t2 - t0 + 63u) >> 6;
-      const auto batchOf = [&](uint32_t r) { return r * nwaves + ((r & 1u) ? nwaves - 1u - wave : wave); };
-      uint32_t r = 0;
-      uint32_t b = batchOf(0);
-      uint32_t kN = t0 + (b << 6) + lane;
-      uint32_t pN = (b < nb && kN < t2) ? primes[kN] : 0u;
-      uint64_t MN = (b < nb && kN < t2) ? magics[kN] : 0ull;
-      for (; b < nb; b = batchOf(r)) {
-        const uint32_t p = pN;
-        const uint64_t M = MN;
-        const uint32_t bn = batchOf(++r);
-        kN = t0 + (bn << 6) + lane;
-        pN = (bn < nb && kN < t2) ? primes[kN] : 0u;
-        MN = (bn < nb && kN < t2) ? magics[kN] : 0ull;
-        if (p) {
-          const uint32_t R = mod_magic(B, p, M);
-          if (nearSmall) strike_medium<true>(p, R, B, S, ctab);
-          else strike_medium<false>(p, R, B, S, ctab);
-        }
-      }
-    }
-
-    // tier 3 (S < p <= bucket threshold): lane = prime, <= 1 hit per class.
-    // The prime itself can only be in the segment while B < p (uniform test
-    // against the largest tier-3 prime).
-    {
-      const uint32_t nb = (t3 - t2 + 63u) >> 6;
+      const uint32_t u1 = (t1 - t0 + 7u) >> 3;
+      const uint32_t u2 = u1 + ((t2 - t1 + 63u) >> 6);
+      const uint32_t u3 = u2 + ((t3 - t2 + 63u) >> 6);
       const uint32_t pTop = (t3 > t2) ? primes[t3 - 1u] : 0u;
       const bool near = B < (uint64_t) pTop;
       const bool small = pTop < (30u << 24);
-      uint32_t b = wave;
-      uint32_t kN = t2 + (b << 6) + lane;
-      uint32_t pN = (b < nb && kN < t3) ? primes[kN] : 0u;
-      uint64_t MN = (b < nb && kN < t3) ? magics[kN] : 0ull;
-      for (; b < nb; b += nwaves) {
-        const uint32_t p = pN;
-        const uint64_t M = MN;
-        const uint32_t bn = b + nwaves;
-        kN = t2 + (bn << 6) + lane;
-        pN = (bn < nb && kN < t3) ? primes[kN] : 0u;
-        MN = (bn < nb && kN < t3) ? magics[kN] : 0ull;
+      const auto primeOf = [&](uint32_t it) { return item_prime(it, lane, t0, t1, t2, t3, u1, u2, u3); };
+      const auto ticket = [=]() -> uint32_t { return (lane == 0) ? atomicAdd(sWork, 1u) : 0u; };
+      const auto uniform = [](uint32_t v) { return (uint32_t) __builtin_amdgcn_readfirstlane((int) v); };
+      uint32_t cur = uniform(ticket());
+      uint32_t nxt = uniform(ticket());
+      uint32_t k = primeOf(cur);
+      uint32_t p = (k != 0xffffffffu) ? primes[k] : 0u;
+      uint64_t M = (k != 0xffffffffu) ? magics[k] : 0ull;
+      while (cur < u3) {
+        const uint32_t after = ticket();                 // the item after nxt, in flight
+        const uint32_t kN = primeOf(nxt);
+        const uint32_t pN = (kN != 0xffffffffu) ? primes[kN] : 0u;
+        const uint64_t MN = (kN != 0xffffffffu) ? magics[kN] : 0ull;
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
-          if (near) strike_big<true, false>(p, R, B, S, ctab);
-          else if (small) strike_big<false, true>(p, R, B, S, ctab);
-          else strike_big<false, false>(p, R, B, S, ctab);
+          if (cur < u1) {
+            const uint32_t i = nearSmall ? class_start<true>(p, R, B, j, ctab)
+                                         : class_start<false>(p, R, B, j, ctab);
+            strike_class(i, p, j, S);
+          } else if (cur < u2) {
+            // lane 0 holds the item's smallest prime
+            const uint32_t pMin = uniform(p);
+            if (4u * pMin > S) {
+              const uint32_t nh = (2u * pMin > S) ? 2u : (3u * pMin > S) ? 3u : 4u;
+              if (nearSmall) strike_few<true>(p, R, B, S, ctab, nh);
+              else strike_few<false>(p, R, B, S, ctab, nh);
+            } else {
+              if (nearSmall) strike_medium<true>(p, R, B, S, ctab);
+              else strike_medium<false>(p, R, B, S, ctab);
+            }
+          } else {
+            if (near) strike_big<true, false>(p, R, B, S, ctab);
+            else if (small) strike_big<false, true>(p, R, B, S, ctab);
+            else strike_big<false, false>(p, R, B, S, ctab);
+          }
         }
+        cur = nxt;
+        nxt = uniform(after);
+        p = pN;
+        M = MN;
       }
     }
   }
 
   // ---- 2b. big-prime bucket list of this segment ------------------------
-  // Entries are bit indices (byte * 8 + bit); four independent loads per
+  // Entries are bit indices (byte * 8 + bit); eight independent loads per
   // thread are in flight before their strikes.
   if (args.buckets) {
     const auto* __restrict__ offs = gptr<uint32_t>(args.listOff);
@@ -490,11 +555,12 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     const uint32_t room = offs[seg + 1u] - e0;
     const uint32_t e1 = e0 + min(gptr<uint32_t>(args.listCount)[seg], room);
     uint32_t e = e0 + tid;
-    for (; e + 3u * nthreads < e1; e += 4u * nthreads) {
-      const uint32_t v0 = ent[e], v1 = ent[e + nthreads];
-      const uint32_t v2 = ent[e + 2u * nthreads], v3 = ent[e + 3u * nthreads];
-      strike_bit(v0); strike_bit(v1);
-      strike_bit(v2); strike_bit(v3);
+    for (; e + 7u * nthreads < e1; e += 8u * nthreads) {
+      uint32_t v[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) v[u] = ent[e + u * nthreads];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) strike_bit(v[u]);
     }
     for (; e < e1; e += nthreads) strike_bit(ent[e]);
   }
