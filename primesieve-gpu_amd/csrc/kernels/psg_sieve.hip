@@ -796,8 +796,8 @@ psg_bucket_fill(PsgBucketArgs a)
   const uint32_t nthreads = PSG_BUCKET_THREADS;        // the launch's block size
   uint32_t* stage = lds;                               // [kStageF][nthreads], then sorted
   uint32_t* hist = stage + PSG_BUCKET_STAGE;           // [W] counts, then scatter cursors
-  // [W] (first sorted slot | fit << 16, entry index of the segment's run)
-  uint2* place = (uint2*) (hist + PSG_MAX_WINDOW_SEGS);
+  // [W] first sorted slot | fit << 16 | entry index of the segment's run << 32
+  uint64_t* place = (uint64_t*) (hist + PSG_MAX_WINDOW_SEGS);
   __shared__ __align__(16) uint32_t ctab[240];
   __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
 
@@ -854,24 +854,45 @@ psg_bucket_fill(PsgBucketArgs a)
     for (uint32_t u = 0; u < kStageF; u++) e[u] = stage[u * nthreads + tid];
     __syncthreads();                                   // cursors set, stage read
 
+    // Scatter by segment, eight hits at a time: their cursor atomics first,
+    // then their stores, so that eight atomics are in flight at once.
 #pragma unroll
-    for (uint32_t u = 0; u < kStageF; u++)
-      if (u < n) stage[atomicAdd(&hist[e[u] >> eshift], 1u)] = e[u];
+    for (uint32_t g = 0; g < kStageF; g += 8) {
+      uint32_t slot[8];
+#pragma unroll
+      for (uint32_t q = 0; q < 8; q++)
+        slot[q] = (g + q < n) ? atomicAdd(&hist[e[g + q] >> eshift], 1u) : 0u;
+#pragma unroll
+      for (uint32_t q = 0; q < 8; q++)
+        if (g + q < n) stage[slot[q]] = e[g + q];
+    }
     if (tid < W) {
       const uint32_t room = listOff[tid + 1u] - listOff[tid];
       if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
       const uint32_t fit = (at >= room) ? 0u : min(cnt, room - at);
-      place[tid] = make_uint2(f | (fit << 16), listOff[tid] + at);
+      place[tid] = (uint64_t) (f | (fit << 16)) | ((uint64_t) (listOff[tid] + at) << 32);
     }
     __syncthreads();
 
+    // Write-out, eight slots at a time: their stage reads, then their
+    // segments' (place, destination) reads, then the stores.  Slots past
+    // `total` hold stale values; their segment index is masked into range
+    // and they store nothing.
 #pragma unroll
-    for (uint32_t u = 0; u < kStageF; u++) {
-      const uint32_t i = u * nthreads + tid;
-      const uint32_t v = stage[i];
-      const uint2 pl = place[(v >> eshift) & (PSG_MAX_WINDOW_SEGS - 1u)];
-      const uint32_t r = i - (pl.x & 0xffffu);
-      if (i < total && r < (pl.x >> 16)) entries[pl.y + r] = v & segBits;
+    for (uint32_t g = 0; g < kStageF; g += 8) {
+      uint32_t v[8];
+      uint64_t pl[8];
+#pragma unroll
+      for (uint32_t q = 0; q < 8; q++) v[q] = stage[(g + q) * nthreads + tid];
+#pragma unroll
+      for (uint32_t q = 0; q < 8; q++) pl[q] = place[(v[q] >> eshift) & (PSG_MAX_WINDOW_SEGS - 1u)];
+#pragma unroll
+      for (uint32_t q = 0; q < 8; q++) {
+        const uint32_t i = (g + q) * nthreads + tid;
+        const uint32_t first = (uint32_t) pl[q], dst = (uint32_t) (pl[q] >> 32);
+        const uint32_t r = i - (first & 0xffffu);
+        if (i < total && r < (first >> 16)) entries[dst + r] = v[q] & segBits;
+      }
     }
     appended += total;
     for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
