@@ -69,13 +69,16 @@ HipBackend::~HipBackend()
   descriptors_.release();
   workCounters_.release();
   listCount_.release();
-  entries_.release();
+  entries_[0].release();
+  entries_[1].release();
   flags_.release();
   if (hostFlags_) (void) api_->HostFree(hostFlags_);
   for (hipEvent_t e : phaseEvents_) (void) api_->EventDestroy(e);
+  if (evSetup_) (void) api_->EventDestroy(evSetup_);
   if (evStart_) (void) api_->EventDestroy(evStart_);
   if (evStop_) (void) api_->EventDestroy(evStop_);
   if (stream_) (void) api_->StreamDestroy(stream_);
+  if (fillStream_) (void) api_->StreamDestroy(fillStream_);
   if (module_) (void) api_->ModuleUnload(module_);
 }
 
@@ -130,6 +133,8 @@ void HipBackend::init()
     throw primesieve_error("psg_sieve_count has " + std::to_string(staticLds) +
                                        " bytes of static LDS; its sieve must start at LDS address 0");
   hipCheck(api_, api_->StreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hipCheck(api_, api_->StreamCreateWithFlags(&fillStream_, hipStreamNonBlocking), "hipStreamCreate");
+  hipCheck(api_, api_->EventCreate(&evSetup_), "hipEventCreate");
   hipCheck(api_, api_->EventCreate(&evStart_), "hipEventCreate");
   hipCheck(api_, api_->EventCreate(&evStop_), "hipEventCreate");
 
@@ -195,7 +200,7 @@ uint32_t HipBackend::primesUpTo(uint64_t x) const
   return (uint32_t) (std::upper_bound(hostPrimes_.begin(), hostPrimes_.end(), v) - hostPrimes_.begin());
 }
 
-void HipBackend::recordPhase(size_t i)
+hipEvent_t HipBackend::phaseEvent(size_t i)
 {
   while (phaseEvents_.size() <= i)
   {
@@ -203,7 +208,7 @@ void HipBackend::recordPhase(size_t i)
     hipCheck(api_, api_->EventCreate(&e), "hipEventCreate");
     phaseEvents_.push_back(e);
   }
-  hipCheck(api_, api_->EventRecord(phaseEvents_[i], stream_), "hipEventRecord");
+  return phaseEvents_[i];
 }
 
 uint32_t HipBackend::persistentGrid(uint32_t nSegs)
@@ -340,13 +345,13 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
     const double E = segExpect_[s];
     room[s] = E > 0 ? (uint32_t) std::min(std::ceil(E + bucketMargin_ * std::sqrt(E) + 64.0), 4.0e9) : 0u;
   }
-  if (!launchPlan(outs, room))
+  if (!launchPlan(outs, room, opt.pipeline))
   {
     // a list outgrew its room: the fill counted every list exactly
     hipCheck(api_, api_->Memcpy(room.data(), listCount_.get(), room.size() * 4, hipMemcpyDeviceToHost),
              "hipMemcpy bucket list counts");
     stats_.bucket_retries = 1;
-    if (!launchPlan(outs, room))
+    if (!launchPlan(outs, room, opt.pipeline))
       throw primesieve_error("big-prime bucket list outgrew its counted room");
   }
   std::copy(hostCounts_.begin(), hostCounts_.end(), out6);
@@ -360,7 +365,7 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
 /// Launch the current plan with the given bucket list rooms (one per
 /// segment of the call).  False when a list outgrew its room (the counts are
 /// then incomplete and listCount_ holds the exact list sizes).
-bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
+bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, bool pipeline)
 {
   const uint32_t S = sieveBytes_;
   uint32_t shift = 0;
@@ -402,8 +407,19 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   counts_.allocate(api_, (size_t) outs * PSG_NCOUNTS * sizeof(uint64_t));
   workCounters_.allocate(api_, slabs_.size() * 4);
   listCount_.allocate(api_, nSegsAll * 4 + 4);
+  // Pipelined (batches): with two or more bucket slabs, the fill of slab k+1
+  // runs on its own stream while slab k is sieved, into the other entry
+  // buffer.  Neither kernel's workgroup fits in LDS beside the other's, so
+  // the fill takes the CUs that the segment kernel's workgroups leave at the
+  // end of a slab: the overlap is the launch tails.
+  size_t bucketSlabs = 0;
+  for (const Slab& sl : slabs_)
+    bucketSlabs += (sl.win1 > sl.win0) ? 1 : 0;
+  const bool pipelined = pipeline && bucketSlabs >= 2;
   if (maxEntries)
-    entries_.allocate(api_, (size_t) maxEntries * 4);
+    entries_[0].allocate(api_, (size_t) maxEntries * 4);
+  if (maxEntries && pipelined)
+    entries_[1].allocate(api_, (size_t) maxEntries * 4);
   flags_.allocate(api_, 16);
   hipCheck(api_, api_->MemsetAsync(counts_.get(), 0, (size_t) outs * PSG_NCOUNTS * 8, stream_), "hipMemsetAsync");
   hipCheck(api_, api_->MemsetAsync(workCounters_.get(), 0, slabs_.size() * 4, stream_), "hipMemsetAsync");
@@ -426,7 +442,6 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   PsgBucketArgs b;
   std::memset(&b, 0, sizeof b);
   b.primes = primes_.addr();
-  b.entries = entries_.addr();
   b.overflow = flags_.addr();
   b.total = flags_.addr() + 8;
   b.segShift = shift;
@@ -434,27 +449,47 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   const uint32_t fillLds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
 
   hipCheck(api_, api_->EventRecord(evStart_, stream_), "hipEventRecord");
+  hipStream_t fs = stream_;
+  if (pipelined)
+  {
+    // the fills start after the copies and clears above
+    fs = fillStream_;
+    hipCheck(api_, api_->EventRecord(evSetup_, stream_), "hipEventRecord");
+    hipCheck(api_, api_->StreamWaitEvent(fs, evSetup_, 0), "hipStreamWaitEvent");
+  }
+  // events: fill begin/end on fs, sieve begin/end on stream_, per slab
+  struct SlabEvents { hipEvent_t fb, fe, sb, se; };
+  std::vector<SlabEvents> ev(slabs_.size(), SlabEvents{ nullptr, nullptr, nullptr, nullptr });
   size_t nev = 0;
-  std::vector<bool> fillPhase;
+  hipEvent_t lastUse[2] = { nullptr, nullptr };   // last sieve reading each entry buffer
+  uint32_t parity = 0;
   for (size_t si = 0; si < slabs_.size(); si++)
   {
     const Slab& sl = slabs_[si];
     const uint64_t slabOff = offBase + offPos[si] * 4;
     const uint64_t slabCount = listCount_.addr() + (uint64_t) sl.segBase * 4;
-    if (sl.win1 > sl.win0)
+    const bool buckets = sl.win1 > sl.win0;
+    if (buckets)
     {
+      if (pipelined && lastUse[parity])
+        hipCheck(api_, api_->StreamWaitEvent(fs, lastUse[parity], 0), "hipStreamWaitEvent");
+      b.entries = entries_[parity].addr();
       b.windows = descriptors_.addr() + runBytes + sl.win0 * sizeof(PsgWindow);
       b.nWindows = (uint32_t) (sl.win1 - sl.win0);
       b.nWGs = sl.nWGs;
       b.listOff = slabOff;
       b.listCount = slabCount;
       void* pb[] = { &b };
-      recordPhase(nev++);
-      fillPhase.push_back(true);
+      ev[si].fb = phaseEvent(nev++);
+      ev[si].fe = phaseEvent(nev++);
+      hipCheck(api_, api_->EventRecord(ev[si].fb, fs), "hipEventRecord");
       hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, sl.nWGs, 1, 1, PSG_BUCKET_THREADS, 1, 1,
-                                              fillLds, stream_, pb, nullptr),
+                                              fillLds, fs, pb, nullptr),
                "hipModuleLaunchKernel(psg_bucket_fill)");
-      args.buckets = entries_.addr();
+      hipCheck(api_, api_->EventRecord(ev[si].fe, fs), "hipEventRecord");
+      if (pipelined)
+        hipCheck(api_, api_->StreamWaitEvent(stream_, ev[si].fe, 0), "hipStreamWaitEvent");
+      args.buckets = entries_[parity].addr();
     }
     else
       args.buckets = 0;
@@ -464,14 +499,20 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
     args.nRuns = (uint32_t) (sl.run1 - sl.run0);
     args.nSegs = sl.nSegs;
     args.workCounter = workCounters_.addr() + si * 4;
-    recordPhase(nev++);
-    fillPhase.push_back(false);
     void* pa[] = { &args };
+    ev[si].sb = phaseEvent(nev++);
+    ev[si].se = phaseEvent(nev++);
+    hipCheck(api_, api_->EventRecord(ev[si].sb, stream_), "hipEventRecord");
     hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, persistentGrid(sl.nSegs), 1, 1, wgSize_, 1, 1,
                                             sieveLds(), stream_, pa, nullptr),
              "hipModuleLaunchKernel(psg_sieve_count)");
+    hipCheck(api_, api_->EventRecord(ev[si].se, stream_), "hipEventRecord");
+    if (buckets)
+    {
+      lastUse[parity] = ev[si].se;
+      if (pipelined) parity ^= 1u;
+    }
   }
-  recordPhase(nev++);
   hipCheck(api_, api_->EventRecord(evStop_, stream_), "hipEventRecord");
   hostCounts_.resize((size_t) outs * PSG_NCOUNTS);
   hipCheck(api_, api_->MemcpyAsync(hostFlags_, flags_.get(), 16, hipMemcpyDeviceToHost, stream_),
@@ -483,12 +524,16 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room)
   float ms = 0.f;
   hipCheck(api_, api_->EventElapsedTime(&ms, evStart_, evStop_), "hipEventElapsedTime");
   double fillMs = 0, sieveMs = 0;
-  for (size_t e = 0; e + 1 < nev; e++)
+  for (const SlabEvents& e : ev)
   {
     float t = 0.f;
-    hipCheck(api_, api_->EventElapsedTime(&t, phaseEvents_[e], phaseEvents_[e + 1]), "hipEventElapsedTime");
-    if (fillPhase[e]) fillMs += t;
-    else sieveMs += t;
+    if (e.fb)
+    {
+      hipCheck(api_, api_->EventElapsedTime(&t, e.fb, e.fe), "hipEventElapsedTime");
+      fillMs += t;
+    }
+    hipCheck(api_, api_->EventElapsedTime(&t, e.sb, e.se), "hipEventElapsedTime");
+    sieveMs += t;
   }
   stats_.kernel_ms += sieveMs;
   stats_.fill_ms += fillMs;
@@ -514,6 +559,7 @@ void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint
   opt.primesPerChunk = 4096;
   opt.maxChunks = 16384;
   opt.slabSegs = std::max<uint32_t>(opt.runSegs, batchSlabSegs_);
+  opt.pipeline = true;
   execute(iv, outs, out6, opt);
 }
 

@@ -65,6 +65,7 @@ struct ExecOptions
   uint32_t primesPerChunk = 512;  // big primes per fill workgroup ...
   uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
   double entryBudget = 1.0e9;     // expected bucket entries per slab (4 GB)
+  bool pipeline = false;          // fill slab k+1 on a second stream while slab k is sieved
 };
 
 class HipBackend : public primesieve::gpu::GpuBackend
@@ -107,7 +108,7 @@ private:
   void ensureSievingPrimes(uint32_t limit);
   void execute(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6,
                const ExecOptions& opt);
-  bool launchPlan(uint32_t outs, const std::vector<uint32_t>& room);
+  bool launchPlan(uint32_t outs, const std::vector<uint32_t>& room, bool pipeline);
   uint32_t primesUpTo(uint64_t x) const;
   uint32_t persistentGrid(uint32_t nSegs);
   // LDS of the segment kernel: the segment, its tail and (when they fit) the
@@ -121,7 +122,7 @@ private:
     return sieveBytes_ + PSG_SIEVE_LDS_TAIL
          + ((config_.preSieve && tilesFitLds()) ? 4u * PSG_SMALL_TILE_WORDS : 0u);
   }
-  void recordPhase(size_t i);
+  hipEvent_t phaseEvent(size_t i);
 
   const HipApi* api_;
   primesieve::gpu::DeviceInfo dev_;
@@ -141,6 +142,8 @@ private:
   hipFunction_t fnSieve_ = nullptr;
   hipFunction_t fnBucketFill_ = nullptr;
   hipStream_t stream_ = nullptr;
+  hipStream_t fillStream_ = nullptr;   // bucket fills of a pipelined call
+  hipEvent_t evSetup_ = nullptr;
   hipEvent_t evStart_ = nullptr;
   hipEvent_t evStop_ = nullptr;
 
@@ -151,7 +154,7 @@ private:
   DeviceBuffer descriptors_;   // runs and windows of every slab of a call
   DeviceBuffer workCounters_;  // uint32 per slab
   DeviceBuffer listCount_;     // uint32 per segment of the call: bucket entries appended
-  DeviceBuffer entries_;       // bucket entries of one slab
+  DeviceBuffer entries_[2];    // bucket entries of a slab; two alternate when pipelined
   DeviceBuffer flags_;         // uint32 overflow flag, pad, uint64 entries appended
   uint64_t* hostFlags_ = nullptr;      // pinned copy of flags_
   std::vector<uint64_t> hostCounts_;

@@ -81,6 +81,7 @@ void load()
   ok &= resolve(lib, "hipStreamCreateWithFlags", a.StreamCreateWithFlags, missing);
   ok &= resolve(lib, "hipStreamDestroy", a.StreamDestroy, missing);
   ok &= resolve(lib, "hipStreamSynchronize", a.StreamSynchronize, missing);
+  ok &= resolve(lib, "hipStreamWaitEvent", a.StreamWaitEvent, missing);
   ok &= resolve(lib, "hipEventCreate", a.EventCreate, missing);
   ok &= resolve(lib, "hipEventRecord", a.EventRecord, missing);
   ok &= resolve(lib, "hipEventSynchronize", a.EventSynchronize, missing);

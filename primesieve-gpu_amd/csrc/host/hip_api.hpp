@@ -49,6 +49,7 @@ struct HipApi
   decltype(&hipStreamCreateWithFlags) StreamCreateWithFlags = nullptr;
   decltype(&hipStreamDestroy) StreamDestroy = nullptr;
   decltype(&hipStreamSynchronize) StreamSynchronize = nullptr;
+  decltype(&hipStreamWaitEvent) StreamWaitEvent = nullptr;
   decltype(&hipEventCreate) EventCreate = nullptr;
   decltype(&hipEventRecord) EventRecord = nullptr;
   decltype(&hipEventSynchronize) EventSynchronize = nullptr;

@@ -41,6 +41,7 @@ hipError_t hipMemsetAsync(void* d, int v, size_t n, void* st) { (void) st; if (!
 hipError_t hipStreamCreateWithFlags(void** s, unsigned f) { (void) f; *s = (void*) 1; return ok; }
 hipError_t hipStreamDestroy(void* s) { (void) s; return ok; }
 hipError_t hipStreamSynchronize(void* s) { (void) s; return ok; }
+hipError_t hipStreamWaitEvent(void* s, void* e, unsigned f) { (void) s; (void) e; (void) f; return ok; }
 hipError_t hipEventCreate(void** e) { *e = (void*) 1; return ok; }
 hipError_t hipEventRecord() { return ok; }
 hipError_t hipEventSynchronize() { return ok; }

@@ -66,3 +66,21 @@ def test_batch_rejects_bad_k(gpu):
 
 def test_empty_batch(gpu):
     assert gpu.count_batch([]) == []
+
+
+def test_batch_over_several_bucket_slabs(gpu):
+    """A batch of about 3 slabs of bucket work: slab k+1 is filled on a second
+    stream while slab k is sieved, with the two entry buffers alternating.
+    Every query must equal its own single-interval call (one stream, no
+    overlap), all six counts."""
+    span = 3000 * 131072 * 30                     # ~3000 segments of 128 KiB
+    starts = [2 * 10 ** 12, 5 * 10 ** 12, 10 ** 13, 3 * 10 ** 13, 7 * 10 ** 13, 10 ** 14]
+    qs = [(s + 7 * i, s + 7 * i + span, k) for i, s in enumerate(starts) for k in range(1, 7)]
+    print("queries", len(qs), "span", span, flush=True)
+    got = gpu.count_batch(qs)
+    st = gpu.last_stats()
+    print("batch stats", st.as_dict(), flush=True)
+    assert st.launches >= 3
+    for i, s in enumerate(starts):
+        want = gpu.count_all(s + 7 * i, s + 7 * i + span)
+        assert got[6 * i:6 * i + 6] == want, (s, got[6 * i:6 * i + 6], want)
