@@ -108,6 +108,19 @@ typedef struct psg_query {
 #define PSG_BATCH_PLAIN 1
 int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int mode);
 
+/* ---- device-resident sieving primes -------------------------------------
+ * No reference counterpart: the reference regenerates its sieving primes on
+ * the CPU whenever sqrt(stop) grows (src/gpu/cuda/CudaBackend.cpp:346-388).
+ * Makes the backend's device cache hold every sieving prime up to `limit`
+ * (from 53 with the pre-sieve, else from 7; a cache reaching 2^24 is
+ * generated on the GPU: the segment kernel counts and then lists the primes
+ * of every segment, and psg_prime_magics adds their division magics) and
+ * copies the cached primes <= limit, ascending, and their magics
+ * ceil(2^(64+L)/p), L = floor(log2 p), read back from the device, to the
+ * caller's arrays (at most cap of each; either may be NULL).  Returns the
+ * number of cached primes <= limit, or PSG_ERROR with errno = EDOM. */
+uint64_t psg_gpu_sieving_primes(uint32_t limit, uint32_t* primes, uint64_t* magics, uint64_t cap);
+
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart.  Statistics of the last count call, measured
  * with HIP events on the backend's own stream. */

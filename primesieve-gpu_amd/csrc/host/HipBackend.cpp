@@ -120,7 +120,8 @@ void HipBackend::init()
   hipCheck(api_, api_->ModuleLoadData(&module_, psg_sieve_hsaco_start),
            ("loading the gfx950 code object (" + std::to_string(coBytes) + " bytes)").c_str());
   const struct { hipFunction_t* f; const char* name; } fns[] = {
-    { &fnSieve_, "psg_sieve_count" }, { &fnBucketFill_, "psg_bucket_fill" } };
+    { &fnSieve_, "psg_sieve_count" }, { &fnBucketFill_, "psg_bucket_fill" },
+    { &fnMagics_, "psg_prime_magics" } };
   for (const auto& f : fns)
     hipCheck(api_, api_->ModuleGetFunction(f.f, module_, f.name),
              (std::string("hipModuleGetFunction(") + f.name + ")").c_str());
@@ -168,6 +169,10 @@ void HipBackend::init()
   hostFlags_[0] = hostFlags_[1] = 0;
 }
 
+// Sieving-prime caches that reach this limit are generated on the device
+// (below it the host's sieve is about as fast as the device round trips).
+constexpr uint32_t kDevicePrimesMin = 1u << 24;
+
 void HipBackend::ensureSievingPrimes(uint32_t limit)
 {
   if (limit <= primesLimit_ && !hostPrimes_.empty())
@@ -176,7 +181,17 @@ void HipBackend::ensureSievingPrimes(uint32_t limit)
   uint64_t want = std::max<uint64_t>(limit, (uint64_t) primesLimit_ * 2);
   want = std::max<uint64_t>(want, 1u << 16);
   want = std::min<uint64_t>(want, 0xffffffffull);
-  std::vector<uint32_t> pr = primesBetween(firstCrossPrime_, (uint32_t) want);
+  if (want < kDevicePrimesMin)
+    installHostPrimes((uint32_t) want);
+  else
+    generatePrimesOnDevice((uint32_t) want);
+}
+
+/// The sieving primes firstCrossPrime_..want from the host's segmented sieve,
+/// with their magics, uploaded to the device cache.
+void HipBackend::installHostPrimes(uint32_t want)
+{
+  std::vector<uint32_t> pr = primesBetween(firstCrossPrime_, want);
   std::vector<uint64_t> mg(pr.size());
   for (size_t i = 0; i < pr.size(); i++)
     mg[i] = divisionMagic(pr[i]);
@@ -191,7 +206,95 @@ void HipBackend::ensureSievingPrimes(uint32_t limit)
   for (size_t i = 0; i < pr.size(); i++)
     invPrefix_[i + 1] = invPrefix_[i] + 1.0 / pr[i];
   hostPrimes_.swap(pr);
-  primesLimit_ = (uint32_t) want;
+  primesLimit_ = want;
+}
+
+/// The sieving primes firstCrossPrime_..want generated on the device: the
+/// host sieves the seed primes up to sqrt(want); the segment kernel counts
+/// the primes of every segment of [firstCrossPrime_, want] (one interval per
+/// segment), the host turns the counts into output offsets, and a second pass
+/// of the same kernel in list mode writes the primes there; psg_prime_magics
+/// adds their magics.  The host keeps a copy of the primes (read back) for
+/// its tier and bucket-room tables.
+void HipBackend::generatePrimesOnDevice(uint32_t want)
+{
+  const psg_stats keep = stats_;
+  const uint32_t S = sieveBytes_;
+  if ((S / 8u) % wgSize_ != 0)
+    throw primesieve_error("device prime generation needs segment bytes divisible by 8 * threads");
+  const uint32_t seed = (uint32_t) std::min<uint64_t>(isqrt64(want) + 1, 0xffffffffull);
+  if (seed > primesLimit_ || hostPrimes_.empty())
+    installHostPrimes(std::max<uint32_t>(seed, 1u << 16));
+
+  // one interval per segment of the global byte grid from firstCrossPrime_ on
+  const uint64_t g0 = (firstCrossPrime_ - 7u) / 30u;
+  std::vector<Interval> iv;
+  for (uint64_t g = g0; ; g += S)
+  {
+    const uint64_t lo = (g == g0) ? firstCrossPrime_ : 30 * g + 7;
+    if (lo > want)
+      break;
+    const uint64_t hi = std::min<uint64_t>(30 * (g + S) + 6, want);
+    iv.push_back(Interval{ lo, hi, (uint32_t) iv.size() });
+  }
+  const uint32_t n = (uint32_t) iv.size();
+  ExecOptions opt;
+  opt.runSegs = 1;
+  opt.slabSegs = std::max<uint32_t>(n, 1);
+  std::vector<uint64_t> six((size_t) n * PSG_NCOUNTS);
+  execute(iv, n, six.data(), opt);                       // pass 1: counts
+
+  std::vector<uint32_t> offsets(n);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; i++)
+  {
+    offsets[i] = (uint32_t) total;
+    total += six[(size_t) i * PSG_NCOUNTS];
+  }
+  DeviceBuffer offs, newPrimes, newMagics;
+  offs.allocate(api_, (size_t) n * 4);
+  newPrimes.allocate(api_, total * 4 + 4);
+  newMagics.allocate(api_, total * 8 + 8);
+  hipCheck(api_, api_->Memcpy(offs.get(), offsets.data(), (size_t) n * 4, hipMemcpyHostToDevice),
+           "hipMemcpy prime offsets");
+  opt.listOut = newPrimes.addr();
+  opt.listOffsets = offs.addr();
+  execute(iv, n, six.data(), opt);                       // pass 2: the primes
+
+  if (total)
+  {
+    uint64_t pa = newPrimes.addr(), ma = newMagics.addr();
+    uint32_t cnt = (uint32_t) total;
+    void* pm[] = { &pa, &ma, &cnt };
+    hipCheck(api_, api_->ModuleLaunchKernel(fnMagics_, (cnt + 255u) / 256u, 1, 1, 256, 1, 1, 0,
+                                            stream_, pm, nullptr), "hipModuleLaunchKernel(psg_prime_magics)");
+  }
+  std::vector<uint32_t> pr(total);
+  hipCheck(api_, api_->StreamSynchronize(stream_), "psg_prime_magics");
+  if (total)
+    hipCheck(api_, api_->Memcpy(pr.data(), newPrimes.get(), total * 4, hipMemcpyDeviceToHost),
+             "hipMemcpy primes to host");
+  primes_.swap(newPrimes);
+  magics_.swap(newMagics);
+  invPrefix_.assign(pr.size() + 1, 0.0);
+  for (size_t i = 0; i < pr.size(); i++)
+    invPrefix_[i + 1] = invPrefix_[i] + 1.0 / pr[i];
+  hostPrimes_.swap(pr);
+  primesLimit_ = want;
+  stats_ = keep;
+}
+
+uint64_t HipBackend::sievingPrimes(uint32_t limit, uint32_t* primes, uint64_t* magics, uint64_t cap)
+{
+  hipCheck(api_, api_->SetDevice(dev_.deviceIndex), "hipSetDevice");
+  ensureSievingPrimes(limit);
+  const uint64_t n = primesUpTo(limit);
+  const uint64_t m = std::min<uint64_t>(n, cap);
+  if (primes && m)
+    std::memcpy(primes, hostPrimes_.data(), m * 4);
+  if (magics && m)
+    hipCheck(api_, api_->Memcpy(magics, magics_.get(), m * 8, hipMemcpyDeviceToHost), "hipMemcpy magics");
+  return n;
 }
 
 uint32_t HipBackend::primesUpTo(uint64_t x) const
@@ -345,13 +448,13 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
     const double E = segExpect_[s];
     room[s] = E > 0 ? (uint32_t) std::min(std::ceil(E + bucketMargin_ * std::sqrt(E) + 64.0), 4.0e9) : 0u;
   }
-  if (!launchPlan(outs, room, opt.pipeline))
+  if (!launchPlan(outs, room, opt))
   {
     // a list outgrew its room: the fill counted every list exactly
     hipCheck(api_, api_->Memcpy(room.data(), listCount_.get(), room.size() * 4, hipMemcpyDeviceToHost),
              "hipMemcpy bucket list counts");
     stats_.bucket_retries = 1;
-    if (!launchPlan(outs, room, opt.pipeline))
+    if (!launchPlan(outs, room, opt))
       throw primesieve_error("big-prime bucket list outgrew its counted room");
   }
   std::copy(hostCounts_.begin(), hostCounts_.end(), out6);
@@ -365,7 +468,7 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
 /// Launch the current plan with the given bucket list rooms (one per
 /// segment of the call).  False when a list outgrew its room (the counts are
 /// then incomplete and listCount_ holds the exact list sizes).
-bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, bool pipeline)
+bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, const ExecOptions& opt)
 {
   const uint32_t S = sieveBytes_;
   uint32_t shift = 0;
@@ -415,7 +518,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, bo
   size_t bucketSlabs = 0;
   for (const Slab& sl : slabs_)
     bucketSlabs += (sl.win1 > sl.win0) ? 1 : 0;
-  const bool pipelined = pipeline && bucketSlabs >= 2;
+  const bool pipelined = opt.pipeline && bucketSlabs >= 2;
   if (maxEntries)
     entries_[0].allocate(api_, (size_t) maxEntries * 4);
   if (maxEntries && pipelined)
@@ -438,6 +541,8 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, bo
   args.tier1End = primesUpTo(std::min<uint32_t>(PSG_TIER1_MAX_PRIME, S));
   args.tier2End = primesUpTo(S);
   args.tier3End = bigBegin_;
+  args.primesOut = opt.listOut;
+  args.outOffsets = opt.listOffsets;
 
   PsgBucketArgs b;
   std::memset(&b, 0, sizeof b);

@@ -23,6 +23,7 @@
 
 #include <stdint.h>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace psg {
@@ -41,6 +42,12 @@ public:
   void* get() const { return ptr_; }
   uint64_t addr() const { return (uint64_t) (uintptr_t) ptr_; }
   size_t bytes() const { return bytes_; }
+  void swap(DeviceBuffer& o)
+  {
+    std::swap(api_, o.api_);
+    std::swap(ptr_, o.ptr_);
+    std::swap(bytes_, o.bytes_);
+  }
 
 private:
   const HipApi* api_ = nullptr;
@@ -66,6 +73,8 @@ struct ExecOptions
   uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
   double entryBudget = 1.0e9;     // expected bucket entries per slab (4 GB)
   bool pipeline = false;          // fill slab k+1 on a second stream while slab k is sieved
+  uint64_t listOut = 0;           // list mode: uint32_t* for the primes (one segment per interval)
+  uint64_t listOffsets = 0;       // list mode: const uint32_t[outs], first output index per interval
 };
 
 class HipBackend : public primesieve::gpu::GpuBackend
@@ -88,6 +97,11 @@ public:
   void countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6);
 
   const psg_stats& lastStats() const { return stats_; }
+  /// Make the device cache hold every sieving prime up to `limit` and copy
+  /// the primes <= limit (up to cap; ascending) and their division magics
+  /// (read back from the device) to the caller's arrays (either may be null).
+  /// Returns the number of cached primes <= limit.
+  uint64_t sievingPrimes(uint32_t limit, uint32_t* primes, uint64_t* magics, uint64_t cap);
 
   /// The geometry actually in use.
   psg_tuning tuning() const
@@ -108,7 +122,9 @@ private:
   void ensureSievingPrimes(uint32_t limit);
   void execute(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6,
                const ExecOptions& opt);
-  bool launchPlan(uint32_t outs, const std::vector<uint32_t>& room, bool pipeline);
+  bool launchPlan(uint32_t outs, const std::vector<uint32_t>& room, const ExecOptions& opt);
+  void installHostPrimes(uint32_t want);
+  void generatePrimesOnDevice(uint32_t want);
   uint32_t primesUpTo(uint64_t x) const;
   uint32_t persistentGrid(uint32_t nSegs);
   // LDS of the segment kernel: the segment, its tail and (when they fit) the
@@ -141,6 +157,7 @@ private:
   hipModule_t module_ = nullptr;
   hipFunction_t fnSieve_ = nullptr;
   hipFunction_t fnBucketFill_ = nullptr;
+  hipFunction_t fnMagics_ = nullptr;
   hipStream_t stream_ = nullptr;
   hipStream_t fillStream_ = nullptr;   // bucket fills of a pipelined call
   hipEvent_t evSetup_ = nullptr;

@@ -183,6 +183,17 @@ int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int m
   }
 }
 
+uint64_t psg_gpu_sieving_primes(uint32_t limit, uint32_t* primes, uint64_t* magics, uint64_t cap)
+{
+  return guarded("psg_gpu_sieving_primes", [&] {
+    std::lock_guard<std::mutex> lock(primesieve::gpu::gpuMutex());
+    psg::HipBackend* b = primesieve::gpu::activeHipBackend();
+    if (!b)
+      throw primesieve::primesieve_error("GPU is not available: " + primesieve::gpu::lastErrorRef());
+    return b->sievingPrimes(limit, primes, magics, cap);
+  });
+}
+
 int psg_gpu_last_stats(psg_stats* out)
 {
   if (!out)

@@ -21,7 +21,7 @@
 
 #define PSG_NCOUNTS 6          /* primes, twins, triplets, ..., sextuplets */
 #define PSG_MAX_SEG_BYTES (160u * 1024u)
-#define PSG_SIEVE_LDS_TAIL 1024u /* LDS after the segment: class table, counters */
+#define PSG_SIEVE_LDS_TAIL 1088u /* LDS after the segment: class table, counters, wave totals */
 /* Pre-sieve tiles: the bit pattern of each group of small primes repeats
  * every P = product-of-the-group bytes.  A tile holds the P sliding 32-bit
  * windows of its pattern (window i = pattern bytes i..i+3, wrapping; a set
@@ -66,6 +66,9 @@ typedef struct PsgSieveArgs {
   uint64_t listOff;     /* const uint32_t[nSegs + 1]: segment s owns entries [listOff[s], listOff[s+1]) */
   uint64_t listCount;   /* const uint32_t[nSegs]: entries appended to segment s (may exceed its room) */
   uint64_t workCounter; /* uint32_t*, zero at launch */
+  uint64_t primesOut;   /* list mode (0 = count mode): uint32_t*; every run is one segment,
+                           and its primes are written ascending from primesOut[outOffsets[run.out]] */
+  uint64_t outOffsets;  /* const uint32_t[outs], list mode */
   uint32_t nRuns;
   uint32_t nSegs;       /* segments in the slab */
   uint32_t segBytes;    /* multiple of 256, <= PSG_MAX_SEG_BYTES */

@@ -406,6 +406,51 @@ __device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint3
   return (it < u3 && k < end) ? k : 0xffffffffu;
 }
 
+// List mode of step 3: write the segment's primes (ascending) instead of
+// counting them.  Thread t owns words64 / nthreads consecutive 64-bit words;
+// an exclusive scan of the per-thread counts (waves by shuffles, the wave
+// totals through LDS) gives each thread its first output index.  The run is
+// one segment, so its output starts at outOffsets[run.out].
+__device__ void list_primes(const PsgSieveArgs& args, const PsgRun& run, uint64_t B,
+                            const uint32_t* sieve, uint32_t* waveTot)
+{
+  const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  const uint64_t* s64 = (const uint64_t*) sieve;
+  const uint32_t per = (args.segBytes >> 3) / nthreads;     // the host checks divisibility
+  const uint32_t w0 = tid * per;
+  const auto word = [&](uint32_t w) {
+    uint64_t x = ~s64[w];
+    const uint64_t gw = B + 8ull * w;
+    if (gw <= run.gLo || gw + 7 >= run.gHi)
+      x = mask_edges(x, gw, run);
+    return x;
+  };
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < per; k++) c += __popcll(word(w0 + k));
+  uint32_t x = c;
+  for (uint32_t d = 1; d < 64u; d <<= 1) {
+    const uint32_t y = (uint32_t) __shfl_up((int) x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63u) waveTot[wave] = x;
+  __syncthreads();
+  uint32_t pos = gptr<uint32_t>(args.outOffsets)[run.out] + x - c;
+  for (uint32_t v = 0; v < wave; v++) pos += waveTot[v];
+  auto* out = gptr_rw<uint32_t>(args.primesOut);
+  constexpr uint64_t kOffsetBytes = 0x1F1D1713110D0B07ull;   // 7, 11, 13, 17, 19, 23, 29, 31
+  for (uint32_t k = 0; k < per; k++) {
+    uint64_t m = word(w0 + k);
+    const uint64_t gw = B + 8ull * (w0 + k);
+    while (m) {
+      const uint32_t b = (uint32_t) __builtin_ctzll(m);
+      m &= m - 1u;
+      const uint64_t v = 30ull * (gw + (b >> 3)) + ((kOffsetBytes >> ((b & 7u) * 8u)) & 0xffu);
+      out[pos++] = (uint32_t) v;
+    }
+  }
+}
+
 // One segment: fill, strike, count.  Ends with a barrier, so that the LDS
 // sieve and the wave sums may be reused by the next segment.
 __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const PsgRun& run,
@@ -566,7 +611,12 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   }
   __syncthreads();
 
-  // ---- 3. count all six types --------------------------------------------
+  // ---- 3. count all six types (list mode: write the primes) --------------
+  if (args.primesOut) {
+    list_primes(args, run, B, sieve, wsum + PSG_NCOUNTS + 3);
+    __syncthreads();
+    return;
+  }
   uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
   {
     const uint64_t* s64 = (const uint64_t*) sieve;
@@ -901,4 +951,23 @@ psg_bucket_fill(PsgBucketArgs a)
   }
   if (tid == 0 && appended)
     atomicAdd((unsigned long long*) a.total, (unsigned long long) appended);
+}
+
+// Division magics of a prime list (the device-resident sieving-prime cache):
+// M = ceil(2^(64+L) / p), L = floor(log2 p), for odd primes p < 2^32, by
+// long division in two 32-bit digits: 2^(32+L) = q1 * p + r (q1 < 2^32,
+// since 2^L < p), then r * 2^32 = q0 * p + r'.  p does not divide 2^(64+L),
+// so the ceiling is the floor plus one.
+extern "C" __global__ void __launch_bounds__(256)
+psg_prime_magics(uint64_t primes, uint64_t magics, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = gptr<uint32_t>(primes)[i];
+  const uint32_t L = 31u - (uint32_t) __clz((int) p);
+  const uint64_t num = 1ull << (32u + L);
+  const uint64_t q1 = num / p;
+  const uint64_t r = num - q1 * p;
+  const uint64_t q0 = (r << 32) / p;
+  gptr_rw<uint64_t>(magics)[i] = (q1 << 32) + q0 + 1u;
 }

@@ -136,6 +136,7 @@ def _declare(lib):
                                 ctypes.POINTER(ctypes.c_float), sz, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]),
         "psg_get_gpu_tuning": (i32, [ctypes.POINTER(Tuning)]),
+        "psg_gpu_sieving_primes": (u64, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, u64]),
     }
     for n in COUNT_NAMES:
         sig["psg_gpu_count_" + n] = (u64, [u64, u64])
@@ -255,6 +256,23 @@ def count_batch(queries, mode=BATCH_PERSISTENT):
     if lib().psg_gpu_count_batch(arr, n, out, mode) != 0:
         raise PrimesieveError(last_error() or "batch count failed")
     return list(out)
+
+
+def sieving_primes(limit):
+    """The device cache's sieving primes <= limit (from 53; from 7 without
+    the pre-sieve) and their division magics, as two numpy arrays (uint32,
+    uint64) read back from the device.  Large caches are generated on the
+    GPU."""
+    import numpy as np
+    L = lib()
+    n = L.psg_gpu_sieving_primes(int(limit), None, None, 0)
+    if n == PSG_ERROR:
+        raise PrimesieveError(last_error() or "sieving primes failed")
+    pr = np.zeros(n, dtype=np.uint32)
+    mg = np.zeros(n, dtype=np.uint64)
+    if n and L.psg_gpu_sieving_primes(int(limit), pr.ctypes.data, mg.ctypes.data, n) != n:
+        raise PrimesieveError(last_error() or "sieving primes failed")
+    return pr, mg
 
 
 def try_count(start, stop, count_type=0):
