@@ -488,7 +488,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
       listOff.push_back((uint32_t) acc);
       acc += room[sl.segBase + s];
     }
-    if (acc > 0xffffffffull)
+    if (acc > 0xffffffffull - (1u << 20))   // entry indices plus a few thread strides stay below 2^32
       throw primesieve_error("big-prime bucket slab exceeds 2^32 entries");
     listOff.push_back((uint32_t) acc);
     maxEntries = std::max(maxEntries, acc);

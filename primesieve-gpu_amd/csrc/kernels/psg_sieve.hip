@@ -488,6 +488,26 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   }
   __syncthreads();
 
+  // The first kBucketPre bucket entries of every thread are loaded now, so
+  // that their HBM latency passes during the strikes (after the fill: its
+  // tile loads would otherwise wait for them).
+  constexpr uint32_t kBucketPre = 16;
+  uint32_t bpre[kBucketPre];
+  uint32_t bE = 0, bE1 = 0;
+  if (args.buckets) {
+    const auto* __restrict__ offs = gptr<uint32_t>(args.listOff);
+    const auto* __restrict__ ent = gptr<uint32_t>(args.buckets);
+    const uint32_t e0 = offs[seg];
+    const uint32_t room = offs[seg + 1u] - e0;
+    bE1 = e0 + min(gptr<uint32_t>(args.listCount)[seg], room);
+    bE = e0 + tid;
+#pragma unroll
+    for (uint32_t u = 0; u < kBucketPre; u++) {
+      const uint32_t idx = bE + u * nthreads;
+      bpre[u] = (idx < bE1) ? ent[idx] : 0u;
+    }
+  }
+
   // ---- 2. strike the sieving primes --------------------------------------
   {
     const uint32_t np = run.nprimes;
@@ -591,15 +611,15 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   }
 
   // ---- 2b. big-prime bucket list of this segment ------------------------
-  // Entries are bit indices (byte * 8 + bit); eight independent loads per
-  // thread are in flight before their strikes.
+  // Entries are bit indices (byte * 8 + bit): the prefetched ones first, then
+  // eight independent loads per thread in flight before their strikes.
   if (args.buckets) {
-    const auto* __restrict__ offs = gptr<uint32_t>(args.listOff);
     const auto* __restrict__ ent = gptr<uint32_t>(args.buckets);
-    const uint32_t e0 = offs[seg];
-    const uint32_t room = offs[seg + 1u] - e0;
-    const uint32_t e1 = e0 + min(gptr<uint32_t>(args.listCount)[seg], room);
-    uint32_t e = e0 + tid;
+    const uint32_t e1 = bE1;
+#pragma unroll
+    for (uint32_t u = 0; u < kBucketPre; u++)
+      if (bE + u * nthreads < e1) strike_bit(bpre[u]);
+    uint32_t e = bE + kBucketPre * nthreads;
     for (; e + 7u * nthreads < e1; e += 8u * nthreads) {
       uint32_t v[8];
 #pragma unroll
