@@ -641,11 +641,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   {
     const uint64_t* s64 = (const uint64_t*) sieve;
     const uint32_t words64 = S >> 3;
-    for (uint32_t w = tid; w < words64; w += nthreads) {
-      uint64_t x = ~s64[w];
-      const uint64_t gw = B + 8ull * w;
-      if (gw <= run.gLo || gw + 7 >= run.gHi)
-        x = mask_edges(x, gw, run);
+    const auto add = [&](uint64_t x) {
       const uint64_t t1 = x & (x >> 1);
       const uint64_t t2 = t1 & (x >> 2);
       const uint64_t t3 = t2 & (x >> 3);
@@ -657,6 +653,25 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
       c3 += __popcll(t3 & 0x0202020202020202ull);   // bits 1..4
       c4 += __popcll(t4 & 0x0303030303030303ull);   // bits 0..4, 1..5
       c5 += __popcll(t5 & 0x0101010101010101ull);   // bits 0..5
+    };
+    uint32_t w = tid;
+    if (B > run.gLo && B + S <= run.gHi) {
+      // the whole segment lies inside the interval: no edge masks, and four
+      // LDS reads in flight per thread
+      for (; w + 3u * nthreads < words64; w += 4u * nthreads) {
+        uint64_t x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) x[u] = ~s64[w + u * nthreads];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) add(x[u]);
+      }
+    }
+    for (; w < words64; w += nthreads) {
+      uint64_t x = ~s64[w];
+      const uint64_t gw = B + 8ull * w;
+      if (gw <= run.gLo || gw + 7 >= run.gHi)
+        x = mask_edges(x, gw, run);
+      add(x);
     }
   }
   c0 = wave_sum(c0); c1 = wave_sum(c1); c2 = wave_sum(c2);
