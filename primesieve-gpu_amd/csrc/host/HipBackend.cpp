@@ -181,7 +181,8 @@ void HipBackend::ensureSievingPrimes(uint32_t limit)
   uint64_t want = std::max<uint64_t>(limit, (uint64_t) primesLimit_ * 2);
   want = std::max<uint64_t>(want, 1u << 16);
   want = std::min<uint64_t>(want, 0xffffffffull);
-  if (want < kDevicePrimesMin)
+  // list mode gives every thread S/8/threads whole 64-bit words
+  if (want < kDevicePrimesMin || (sieveBytes_ / 8u) % wgSize_ != 0)
     installHostPrimes((uint32_t) want);
   else
     generatePrimesOnDevice((uint32_t) want);
@@ -220,8 +221,6 @@ void HipBackend::generatePrimesOnDevice(uint32_t want)
 {
   const psg_stats keep = stats_;
   const uint32_t S = sieveBytes_;
-  if ((S / 8u) % wgSize_ != 0)
-    throw primesieve_error("device prime generation needs segment bytes divisible by 8 * threads");
   const uint32_t seed = (uint32_t) std::min<uint64_t>(isqrt64(want) + 1, 0xffffffffull);
   if (seed > primesLimit_ || hostPrimes_.empty())
     installHostPrimes(std::max<uint32_t>(seed, 1u << 16));

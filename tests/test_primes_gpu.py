@@ -57,3 +57,22 @@ def test_device_primes_to_1e9(gpu, oracle):
     # the counts that use these primes: a window at 1e18 against the oracle
     a = 10 ** 18
     assert gpu.count_all(a, a + 10 ** 6) == oracle.count6(a, a + 10 ** 6)
+
+
+@pytest.mark.parametrize("kb,wg,presieve", [(32, 512, True), (150, 256, True), (128, 1024, False)])
+def test_device_primes_other_geometries(gpu, kb, wg, presieve):
+    """A new tuning makes a new backend and a new cache: generated with other
+    segment and workgroup sizes (and without the pre-sieve, from 7) it holds
+    the same primes."""
+    limit = (1 << 24) + 999
+    want, _ = gpu.sieving_primes(limit)
+    gpu.set_tuning(sieve_bytes=kb * 1024, work_group_size=wg, presieve=presieve)
+    try:
+        pr, mg = gpu.sieving_primes(limit)
+        print("geometry", kb, wg, presieve, "primes", len(pr), flush=True)
+        if not presieve:
+            assert list(pr[:12]) == [7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47]
+            pr, mg = pr[12:], mg[12:]
+        assert np.array_equal(pr, want)
+    finally:
+        gpu.set_tuning()
