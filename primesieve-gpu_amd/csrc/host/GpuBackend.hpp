@@ -45,7 +45,7 @@ struct GpuConfig
   bool preSieve = true;            // pre-sieve tiles for 7..47
   uint32_t slabSegments = 0;       // segments per big-prime bucket slab
   uint32_t bucketFactor = 0;       // primes > bucketFactor * sieveBytes use the
-                                   // bucket tier; 0 = auto (8), ~0u = never
+                                   // bucket tier; 0 = auto (7), ~0u = never
   uint32_t bucketMargin = 0;       // bucket list room beyond the expected count,
                                    // in standard deviations; 0 = auto (6)
 };

@@ -105,7 +105,7 @@ void HipBackend::init()
   wgSize_ = wg;
 
   segmentsPerLaunch_ = config_.segmentsPerLaunch ? config_.segmentsPerLaunch : 65536u;
-  bucketFactor_ = (config_.bucketFactor == 0) ? 8u
+  bucketFactor_ = (config_.bucketFactor == 0) ? 7u
                 : (config_.bucketFactor == 0xffffffffu) ? 0u : config_.bucketFactor;
   // a slab spans ~1 GiB of sieve bytes (fewer launch tails; its bucket
   // entries, ~4 B per crossing of a big prime, fit easily in HBM)
