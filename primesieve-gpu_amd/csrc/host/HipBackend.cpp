@@ -107,11 +107,12 @@ void HipBackend::init()
   segmentsPerLaunch_ = config_.segmentsPerLaunch ? config_.segmentsPerLaunch : 65536u;
   bucketFactor_ = (config_.bucketFactor == 0) ? 7u
                 : (config_.bucketFactor == 0xffffffffu) ? 0u : config_.bucketFactor;
-  // a slab spans ~1 GiB of sieve bytes (fewer launch tails; its bucket
-  // entries, ~4 B per crossing of a big prime, fit easily in HBM)
+  // a slab spans up to 4 GiB of sieve bytes (one launch of each kernel
+  // for a 1e11 interval: no launch tails between slabs; its bucket entries,
+  // 4 B per crossing of a big prime and at most the entry budget, fit in HBM)
   uint32_t slab = config_.slabSegments ? config_.slabSegments
-                                       : (uint32_t) ((1ull << 30) / sieveBytes_);
-  slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 8192));
+                                       : (uint32_t) ((1ull << 32) / sieveBytes_);
+  slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 32768));
   windowSegs_ = 128;
   bucketMargin_ = config_.bucketMargin ? config_.bucketMargin : 6u;
   firstCrossPrime_ = config_.preSieve ? PSG_FIRST_CROSS_PRIME : 7u;

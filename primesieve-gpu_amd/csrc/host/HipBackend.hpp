@@ -71,7 +71,7 @@ struct ExecOptions
   uint32_t windowSegs = 64;       // segments per bucket window
   uint32_t primesPerChunk = 512;  // big primes per fill workgroup ...
   uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
-  double entryBudget = 1.0e9;     // expected bucket entries per slab (4 GB)
+  double entryBudget = 3.0e9;     // expected bucket entries per slab (12 GB)
   bool pipeline = false;          // fill slab k+1 on a second stream while slab k is sieved
   uint64_t listOut = 0;           // list mode: uint32_t* for the primes (one segment per interval)
   uint64_t listOffsets = 0;       // list mode: const uint32_t[outs], first output index per interval
