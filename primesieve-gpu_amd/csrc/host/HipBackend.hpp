@@ -69,7 +69,7 @@ struct ExecOptions
   uint32_t runSegs = 1024;        // longest run
   uint32_t slabSegs = 1024;       // segments per slab (one persistent launch)
   uint32_t windowSegs = 64;       // segments per bucket window
-  uint32_t primesPerChunk = 512;  // big primes per fill workgroup ...
+  uint32_t primesPerChunk = 1024; // big primes per fill workgroup ...
   uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
   double entryBudget = 3.0e9;     // expected bucket entries per slab (12 GB)
   bool pipeline = false;          // fill slab k+1 on a second stream while slab k is sieved
