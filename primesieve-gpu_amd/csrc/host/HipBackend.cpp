@@ -87,13 +87,13 @@ void HipBackend::init()
   hipCheck(api_, api_->SetDevice(dev_.deviceIndex), "hipSetDevice");
 
   // geometry: LDS segment, workgroup, launch size
-  const uint64_t ldsMax = std::min<uint64_t>(dev_.localMemBytes, PSG_MAX_SEG_BYTES + 1024);
+  const uint64_t ldsMax = std::min<uint64_t>(dev_.localMemBytes, PSG_MAX_SEG_BYTES + PSG_SIEVE_LDS_TAIL);
   ldsBytes_ = (uint32_t) ldsMax;
   uint32_t sb = config_.sieveBytes ? config_.sieveBytes : 128u * 1024u;
   sb = (sb / 256u) * 256u;
   if (sb < 256u) sb = 256u;
   if (sb > PSG_MAX_SEG_BYTES) sb = PSG_MAX_SEG_BYTES;
-  while (sb > 256u && (uint64_t) sb + 1024u > ldsMax) sb -= 256u;
+  while (sb > 256u && (uint64_t) sb + PSG_SIEVE_LDS_TAIL > ldsMax) sb -= 256u;  // the segment and its tail
   sieveBytes_ = sb;
 
   uint32_t wg = config_.workGroupSize ? config_.workGroupSize : 1024u;

@@ -90,7 +90,7 @@ def test_random_ranges_golden(gpu, golden):
     assert not bad, bad[:5]
 
 
-@pytest.mark.parametrize("kb,wg", [(16, 256), (32, 512), (64, 1024), (128, 512), (150, 256)])
+@pytest.mark.parametrize("kb,wg", [(16, 256), (32, 512), (64, 1024), (128, 512), (150, 256), (160, 1024)])
 def test_geometries_against_oracle(gpu, oracle, kb, wg):
     """Every segment size / workgroup size gives the oracle's counts,
     including ranges that straddle segment ends and byte ends."""
