@@ -33,7 +33,9 @@
 //              plus the bucket list of the segment, filled by
 //              psg_bucket_fill for larger primes (one LDS atomic per entry);
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
-//              pass; wave sums use ds_swizzle butterflies.
+//              pass; wave sums use ds_swizzle butterflies.  In list mode the
+//              segment's primes are written out instead (the device's
+//              sieving-prime cache; psg_prime_magics adds their magics).
 //
 // In LDS a set bit marks a composite (the counts read the complement), so a
 // strike is a ds_or_b32 atomic with a one-bit mask: several lanes may hit
