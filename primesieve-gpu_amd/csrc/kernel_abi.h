@@ -21,7 +21,8 @@
 
 #define PSG_NCOUNTS 6          /* primes, twins, triplets, ..., sextuplets */
 #define PSG_MAX_SEG_BYTES (160u * 1024u)
-#define PSG_SIEVE_LDS_TAIL 1088u /* LDS after the segment: class table, counters, wave totals */
+#define PSG_SIEVE_LDS_TAIL 1856u /* LDS after the segment: class table, counters, wave totals,
+                                    tier-0 primes and magics */
 /* Pre-sieve tiles: the bit pattern of each group of small primes repeats
  * every P = product-of-the-group bytes.  A tile holds the P sliding 32-bit
  * windows of its pattern (window i = pattern bytes i..i+3, wrapping; a set
@@ -39,6 +40,7 @@
 #define PSG_FIRST_CROSS_PRIME 53u
 #define PSG_TIER0_MAX_PRIME 256u
 #define PSG_TIER1_MAX_PRIME 2048u
+#define PSG_TIER0_SLOTS 64u      /* LDS slots for the tier-0 primes (>= the 51 primes 7..251) */
 
 /* One run (48 bytes). */
 typedef struct PsgRun {

@@ -26,12 +26,14 @@
 //                        stride 4p with loop-invariant masks;
 //                tier 2 (2048 < p <= S): lane = prime, 64 per item, the same
 //                        progressions for each of its classes;
-//                tier 3 (S < p <= bucket threshold): lane = prime, at most
-//                        one hit per class;
+//                tier 3 (S < p <= bucket threshold): 4 primes per lane, 256
+//                        per item, at most one hit per class;
 //              the items of tiers 1-3 form one queue per segment that the
-//              waves take from, heaviest first;
+//              waves take from: tiers 1-2 heaviest first, the light tier-3
+//              items spread evenly between them;
 //              plus the bucket list of the segment, filled by
-//              psg_bucket_fill for larger primes (one LDS atomic per entry);
+//              psg_bucket_fill for larger primes (one LDS atomic per entry),
+//              whose entries are loaded while the queue's items are struck;
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
 //              pass; wave sums use ds_swizzle butterflies.  In list mode the
 //              segment's primes are written out instead (the device's
@@ -239,15 +241,14 @@ __device__ __forceinline__ void strike_few(uint32_t p, uint32_t R, uint64_t B, u
   }
 }
 
-// The <= 8 hits of a prime S < p in the segment at B (R = B mod p): one per
-// residue class at most.  kSmall: p < 30 * 2^24, so that (p/30) * q_j is a
-// 24-bit multiply (and p < 2^31, which sub_mod needs).
+// The <= 8 hits of a prime S < p in the segment at B (R = B mod p, a = p/30,
+// row = its class entries): one per residue class at most.  kSmall:
+// p < 30 * 2^24, so that (p/30) * q_j is a 24-bit multiply (and p < 2^31,
+// which sub_mod needs).
 template <bool kNear, bool kSmall>
-__device__ __forceinline__ void strike_big(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
-                                           const uint32_t* ctab)
+__device__ __forceinline__ void strike_big(uint32_t p, uint32_t a, uint32_t R, uint64_t B,
+                                           uint32_t S, const ClassRow& row)
 {
-  const uint32_t a = p / 30u;
-  const ClassRow row = class_row(ctab, p - a * 30u);
   const uint32_t R1 = R + 1u;
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
@@ -263,6 +264,26 @@ __device__ __forceinline__ void strike_big(uint32_t p, uint32_t R, uint64_t B, u
       i = 0xffffffffu;                  // the prime itself; its next hit is beyond S
     if (i < S) lds_or(lds_word(i), 1u << (((i & 3u) << 3) + c));
   }
+}
+
+// strike_big for the kN primes of a lane (p[m] = 0: none).  All their class
+// rows are read before the first strike: the LDS operations of a wave
+// complete in order, so a row read issued after strikes would wait for all
+// of them.
+template <bool kNear, bool kSmall, uint32_t kN>
+__device__ __forceinline__ void strike_big_lane(const uint32_t (&p)[kN], const uint32_t (&R)[kN],
+                                                uint64_t B, uint32_t S, const uint32_t* ctab)
+{
+  uint32_t a[kN];
+  ClassRow row[kN];
+#pragma unroll
+  for (uint32_t m = 0; m < kN; m++) {
+    a[m] = p[m] / 30u;
+    row[m] = class_row(ctab, p[m] - a[m] * 30u);
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < kN; m++)
+    if (p[m]) strike_big<kNear, kSmall>(p[m], a[m], R[m], B, S, row[m]);
 }
 
 // Strike bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
@@ -386,10 +407,25 @@ __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint
   for (; w < words; w += nthreads) sieve[w] = word(w);
 }
 
-// Prime index of lane `lane` in work item `it` of a segment's queue (see
-// sieve_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each (lane =
-// prime * 8 + class), [u1, u2) and [u2, u3) 64 primes of [t1, t2) and
-// [t2, t3) (lane = prime).  `it` is uniform.
+// Tier-3 primes per lane: a tier-3 item holds 64 * kT3 primes, so that the
+// per-item costs (the queue ticket, the wait for the previous item's strikes
+// before its class rows can be read) are shared by kT3 primes.
+constexpr uint32_t kT3 = 4;
+
+// LDS after the segment: the class table, the workgroup's counters (the six
+// sums, segment and run numbers, the queue counter, the list mode's wave
+// totals), then the tier-0 primes and their magics (read once per workgroup).
+constexpr uint32_t kCtabWords = 240;
+constexpr uint32_t kVarWords = 32;
+constexpr uint32_t kTier0Slots = PSG_TIER0_SLOTS;
+static_assert((kCtabWords + kVarWords + 3 * kTier0Slots) * 4 == PSG_SIEVE_LDS_TAIL,
+              "LDS tail layout");
+
+// First prime index of lane `lane` in work item `it` of a segment's queue
+// (see sieve_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each
+// (lane = prime * 8 + class), [u1, u2) 64 primes of [t1, t2) (lane = prime),
+// [u2, u3) 64 * kT3 primes of [t2, t3) (lane m-th prime: index + 64 m).  `it`
+// is uniform.
 __device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint32_t t0, uint32_t t1,
                                                uint32_t t2, uint32_t t3, uint32_t u1, uint32_t u2,
                                                uint32_t u3)
@@ -402,7 +438,7 @@ __device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint3
     k = t1 + ((it - u1) << 6) + lane;
     end = t2;
   } else {
-    k = t2 + ((it - u2) << 6) + lane;
+    k = t2 + (it - u2) * (64u * kT3) + lane;
     end = t3;
   }
   return (it < u3 && k < end) ? k : 0xffffffffu;
@@ -491,8 +527,9 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   __syncthreads();
 
   // The first kBucketPre bucket entries of every thread are loaded now, so
-  // that their HBM latency passes during the strikes (after the fill: its
-  // tile loads would otherwise wait for them).
+  // that their HBM latency passes during tier 0, which waits on no global
+  // load (after the fill: its tile loads would otherwise wait for them).
+  // They are struck when the queue starts.
   constexpr uint32_t kBucketPre = 16;
   uint32_t bpre[kBucketPre];
   uint32_t bE = 0, bE1 = 0;
@@ -524,16 +561,20 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     const bool nearSmall = B < (uint64_t) S;   // tiers 0-2 hold primes <= S
 
     // tier 0 (p <= 256): every wave strikes all of them over its own slice
-    // of the segment, so the heaviest primes are shared evenly.
+    // of the segment, so the heaviest primes are shared evenly.  Their primes
+    // and magics come from the workgroup's LDS copy: no global load waits
+    // here, so the bucket entries loaded above arrive meanwhile.
     {
+      const uint32_t* t0p = ctab + kCtabWords + kVarWords;
+      const uint64_t* t0m = (const uint64_t*) (t0p + kTier0Slots);
       const uint32_t sliceBytes = S / nwaves;          // S is a multiple of 256
       const uint32_t sliceLo = wave * sliceBytes;
       const uint32_t sliceHi = sliceLo + sliceBytes;
       const uint64_t Bs = B + sliceLo;
       for (uint32_t b = 0; b < t0; b += 8) {
-        const uint32_t k = b + slot;
-        const uint32_t p = (k < t0) ? primes[k] : 0u;
-        const uint32_t R = p ? mod_magic(Bs, p, magics[k]) : 0u;
+        const uint32_t k = b + slot;                     // t0 <= kTier0Slots
+        const uint32_t p = (k < t0) ? t0p[k] : 0u;
+        const uint32_t R = p ? mod_magic(Bs, p, t0m[k]) : 0u;
         const uint32_t st = !p ? 0u : nearSmall ? class_start<true>(p, R, Bs, j, ctab)
                                                : class_start<false>(p, R, Bs, j, ctab);
         // all eight permutes first: one wait for the batch, not one per prime
@@ -550,37 +591,95 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     }
 
     // tiers 1-3: a queue of work items shared by the waves of the workgroup
-    // (an LDS counter), heaviest first, so that no wave is left with a long
-    // tail of work while the others wait at the barrier:
+    // (an LDS counter), so that no wave is left with a long tail of work
+    // while the others wait at the barrier:
     //   tier 1 (256 < p <= 2048): 8 primes per item, lane = (prime, class),
     //     each lane strikes one class over the whole segment (the primes of
     //     an item are close, so its lanes finish together);
     //   tier 2 (2048 < p <= S): 64 primes per item, lane = prime;
-    //   tier 3 (S < p <= bucket threshold): 64 primes per item, lane = prime,
-    //     <= 1 hit per class; the prime itself can only be in the segment
-    //     while B < p (uniform test against the largest tier-3 prime).
-    // The next item's number and primes are fetched while the current item
-    // is struck.
+    //   tier 3 (S < p <= bucket threshold): 64 * kT3 primes per item, kT3 per
+    //     lane, <= 1 hit per class; the prime itself can only be in the
+    //     segment while B < p (uniform test against the largest tier-3 prime).
+    // Ticket t maps to an item so that tiers 1-2 keep their order (heaviest
+    // first) and the light tier-3 items are spread evenly between them: waves
+    // on LDS-bound strikes then share the CU with waves computing tier-3
+    // class starts.  With g(t) = floor(t * n3 / u3) tier-3 items before
+    // ticket t (n3 = u3 - u2; computed as (t * r) >> 32 with
+    // r = ceil(n3 * 2^32 / u3), which is exact at t = u3 and steps by at most
+    // one), ticket t is tier-3 item g(t) when g(t + 1) > g(t), otherwise item
+    // t - g(t) of tiers 1-2.
+    // While an item is struck, the next item's number and primes are in
+    // flight, and so is the thread's next batch of 8 bucket entries: a batch
+    // is struck at the start of the item after the one it was loaded in.
     {
       const uint32_t u1 = (t1 - t0 + 7u) >> 3;
       const uint32_t u2 = u1 + ((t2 - t1 + 63u) >> 6);
-      const uint32_t u3 = u2 + ((t3 - t2 + 63u) >> 6);
+      const uint32_t u3 = u2 + (t3 - t2 + 64u * kT3 - 1u) / (64u * kT3);
       const uint32_t pTop = (t3 > t2) ? primes[t3 - 1u] : 0u;
       const bool near = B < (uint64_t) pTop;
       const bool small = pTop < (30u << 24);
+      const uint64_t rmix = (((uint64_t) (u3 - u2) << 32) + u3 - 1u) / max(u3, 1u);
+      const auto itemOf = [=](uint32_t t) -> uint32_t {
+        if (t >= u3) return t;
+        const uint32_t g0 = (uint32_t) (((uint64_t) t * rmix) >> 32);
+        const uint32_t g1 = (uint32_t) (((uint64_t) (t + 1u) * rmix) >> 32);
+        return (g1 > g0) ? u2 + g0 : t - g0;
+      };
       const auto primeOf = [&](uint32_t it) { return item_prime(it, lane, t0, t1, t2, t3, u1, u2, u3); };
       const auto ticket = [=]() -> uint32_t { return (lane == 0) ? atomicAdd(sWork, 1u) : 0u; };
       const auto uniform = [](uint32_t v) { return (uint32_t) __builtin_amdgcn_readfirstlane((int) v); };
-      uint32_t cur = uniform(ticket());
-      uint32_t nxt = uniform(ticket());
+      uint32_t cur = itemOf(uniform(ticket()));
+      uint32_t nxt = itemOf(uniform(ticket()));
+      // the lane's first prime of the current item, and its further tier-3 primes
       uint32_t k = primeOf(cur);
       uint32_t p = (k != 0xffffffffu) ? primes[k] : 0u;
       uint64_t M = (k != 0xffffffffu) ? magics[k] : 0ull;
+      uint32_t pX[kT3 - 1];
+      uint64_t MX[kT3 - 1];
+#pragma unroll
+      for (uint32_t m = 0; m < kT3 - 1; m++) {
+        const uint32_t km = k + 64u * (m + 1u);
+        const bool ok = cur >= u2 && k != 0xffffffffu && km < t3;
+        pX[m] = ok ? primes[km] : 0u;
+        MX[m] = ok ? magics[km] : 0ull;
+      }
+      // the bucket entries prefetched before tier 0 (arrived by now), then
+      // the first batch in flight; a lane loads min(index, last) so that its
+      // loads stay inside the list
+      const auto* __restrict__ bent = gptr<uint32_t>(args.buckets);
+#pragma unroll
+      for (uint32_t u = 0; u < kBucketPre; u++)
+        if (bE + u * nthreads < bE1) strike_bit(bpre[u]);
+      bE += kBucketPre * nthreads;
+      uint32_t bq[8];
+      if (bE < bE1) {
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) bq[u] = bent[min(bE + u * nthreads, bE1 - 1u)];
+      }
       while (cur < u3) {
         const uint32_t after = ticket();                 // the item after nxt, in flight
+        if (bE < bE1) {
+#pragma unroll
+          for (uint32_t u = 0; u < 8; u++)
+            if (bE + u * nthreads < bE1) strike_bit(bq[u]);
+          bE += 8u * nthreads;
+          if (bE < bE1) {
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) bq[u] = bent[min(bE + u * nthreads, bE1 - 1u)];
+          }
+        }
         const uint32_t kN = primeOf(nxt);
         const uint32_t pN = (kN != 0xffffffffu) ? primes[kN] : 0u;
         const uint64_t MN = (kN != 0xffffffffu) ? magics[kN] : 0ull;
+        uint32_t pNX[kT3 - 1];
+        uint64_t MNX[kT3 - 1];
+#pragma unroll
+        for (uint32_t m = 0; m < kT3 - 1; m++) {
+          const uint32_t km = kN + 64u * (m + 1u);
+          const bool ok = nxt >= u2 && kN != 0xffffffffu && km < t3;
+          pNX[m] = ok ? primes[km] : 0u;
+          MNX[m] = ok ? magics[km] : 0ull;
+        }
         if (p) {
           const uint32_t R = mod_magic(B, p, M);
           if (cur < u1) {
@@ -599,29 +698,46 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
               else strike_medium<false>(p, R, B, S, ctab);
             }
           } else {
-            if (near) strike_big<true, false>(p, R, B, S, ctab);
-            else if (small) strike_big<false, true>(p, R, B, S, ctab);
-            else strike_big<false, false>(p, R, B, S, ctab);
+            uint32_t pa[kT3], Ra[kT3];
+            pa[0] = p;
+            Ra[0] = R;
+#pragma unroll
+            for (uint32_t m = 0; m < kT3 - 1; m++) {
+              pa[m + 1] = pX[m];
+              Ra[m + 1] = pX[m] ? mod_magic(B, pX[m], MX[m]) : 0u;
+            }
+            if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctab);
+            else if (small) strike_big_lane<false, true, kT3>(pa, Ra, B, S, ctab);
+            else strike_big_lane<false, false, kT3>(pa, Ra, B, S, ctab);
           }
         }
         cur = nxt;
-        nxt = uniform(after);
+        nxt = itemOf(uniform(after));
         p = pN;
         M = MN;
+#pragma unroll
+        for (uint32_t m = 0; m < kT3 - 1; m++) {
+          pX[m] = pNX[m];
+          MX[m] = MNX[m];
+        }
+      }
+      if (bE < bE1) {
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++)
+          if (bE + u * nthreads < bE1) strike_bit(bq[u]);
+        bE += 8u * nthreads;
       }
     }
   }
 
-  // ---- 2b. big-prime bucket list of this segment ------------------------
-  // Entries are bit indices (byte * 8 + bit): the prefetched ones first, then
-  // eight independent loads per thread in flight before their strikes.
+  // ---- 2b. the rest of the segment's bucket list -------------------------
+  // Entries are bit indices (byte * 8 + bit).  Most were struck during the
+  // queue; the rest (short queues) with eight independent loads per thread in
+  // flight before their strikes.
   if (args.buckets) {
     const auto* __restrict__ ent = gptr<uint32_t>(args.buckets);
     const uint32_t e1 = bE1;
-#pragma unroll
-    for (uint32_t u = 0; u < kBucketPre; u++)
-      if (bE + u * nthreads < e1) strike_bit(bpre[u]);
-    uint32_t e = bE + kBucketPre * nthreads;
+    uint32_t e = bE;
     for (; e + 7u * nthreads < e1; e += 8u * nthreads) {
       uint32_t v[8];
 #pragma unroll
@@ -831,8 +947,8 @@ psg_sieve_count(PsgSieveArgs args)
   // class table and the block's small variables follow it
   extern __shared__ uint32_t lds[];
   uint32_t* sieve = lds;
-  uint32_t* ctab = lds + (args.segBytes >> 2);                 // 240 entries, 16-byte aligned
-  uint32_t* wsum = ctab + 240u;                                 // PSG_NCOUNTS
+  uint32_t* ctab = lds + (args.segBytes >> 2);                 // 16-byte aligned
+  uint32_t* wsum = ctab + kCtabWords;                           // PSG_NCOUNTS
   uint32_t& sSeg = wsum[PSG_NCOUNTS];
   uint32_t& sRun = wsum[PSG_NCOUNTS + 1];
   // the small pre-sieve tiles, copied once per workgroup when they fit
@@ -846,6 +962,17 @@ psg_sieve_count(PsgSieveArgs args)
 
   const auto* runs = gptr<PsgRun>(args.runs);
   init_ctab(ctab);
+  {
+    uint32_t* t0p = ctab + kCtabWords + kVarWords;
+    uint64_t* t0m = (uint64_t*) (t0p + kTier0Slots);
+    const auto* pr = gptr<uint32_t>(args.primes);
+    const auto* mg = gptr<uint64_t>(args.magics);
+    const uint32_t n0 = min(args.tier0End, kTier0Slots);
+    for (uint32_t i = threadIdx.x; i < kTier0Slots; i += blockDim.x) {
+      t0p[i] = (i < n0) ? pr[i] : 0u;
+      t0m[i] = (i < n0) ? mg[i] : 0ull;
+    }
+  }
   for (;;) {
     if (threadIdx.x == 0) {
       const uint32_t seg = atomicAdd((uint32_t*) args.workCounter, 1u);
