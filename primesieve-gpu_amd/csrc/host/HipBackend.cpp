@@ -152,11 +152,12 @@ void HipBackend::init()
     {
       const std::vector<uint32_t> w = tileWindows(groups[t]);
       const uint32_t Q = rows[t];
-      if (w.size() != periods[t] || all.size() != offsets[t] || Q != (periods[t] + 3) / 4)
+      if (w.size() != periods[t] || all.size() != offsets[t] || Q != (periods[t] + 3) / 4 + 3)
         throw primesieve_error("pre-sieve tile layout mismatch");
       std::vector<uint32_t> tile((size_t) 4 * Q, 0);
-      for (uint32_t i = 0; i < periods[t]; i++)
-        tile[(size_t) (i % 4) * Q + i / 4] = ~w[i];      // the kernel marks composites
+      for (uint32_t r = 0; r < 4; r++)
+        for (uint32_t j = 0; j < Q; j++)
+          tile[(size_t) r * Q + j] = ~w[(r + 4ull * j) % periods[t]];      // the kernel marks composites
       all.insert(all.end(), tile.begin(), tile.end());
     }
     if (all.size() != PSG_TILE_WORDS)

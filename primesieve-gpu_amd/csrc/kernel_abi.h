@@ -27,16 +27,18 @@
  * every P = product-of-the-group bytes.  A tile holds the P sliding 32-bit
  * windows of its pattern (window i = pattern bytes i..i+3, wrapping; a set
  * bit marks a multiple of a tile prime, as in the kernel's sieve) in
- * residue-major order: window i at word (i % 4) * Q + i / 4, Q = ceil(P/4),
- * so the windows 4 apart that consecutive sieve words need are adjacent.
+ * residue-major order: 4 rows of L = ceil(P/4) + 3 words, word j of row r
+ * holding window (r + 4j) mod P.  Window i < P sits at row i % 4, word i / 4,
+ * and the 4 windows i, i+4, i+8, i+12 that 4 consecutive sieve words need
+ * follow it in the same row (each row runs 3 windows past the period).
  * Tiles are stored one after another; tiles 1.. (PSG_SMALL_TILE_WORDS
  * words) are copied into LDS when the segment leaves room. */
 #define PSG_NTILES 5
 #define PSG_TILE_PERIODS { 17017u, 437u, 899u, 1517u, 2021u }   /* 7*11*13*17, 19*23, 29*31, 37*41, 43*47 */
-#define PSG_TILE_ROWS { 4255u, 110u, 225u, 380u, 506u }         /* Q = ceil(P / 4) */
-#define PSG_TILE_OFFSETS { 0u, 17020u, 17460u, 18360u, 19880u } /* first word of each tile */
-#define PSG_TILE_WORDS 21904u
-#define PSG_SMALL_TILE_WORDS (PSG_TILE_WORDS - 17020u)
+#define PSG_TILE_ROWS { 4258u, 113u, 228u, 383u, 509u }         /* L = ceil(P / 4) + 3 */
+#define PSG_TILE_OFFSETS { 0u, 17032u, 17484u, 18396u, 19928u } /* first word of each tile */
+#define PSG_TILE_WORDS 21964u
+#define PSG_SMALL_TILE_WORDS (PSG_TILE_WORDS - 17032u)
 #define PSG_FIRST_CROSS_PRIME 53u
 #define PSG_TIER0_MAX_PRIME 256u
 #define PSG_TIER1_MAX_PRIME 2048u

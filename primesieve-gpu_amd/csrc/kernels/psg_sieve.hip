@@ -368,8 +368,11 @@ __device__ void init_ctab(uint32_t* ctab)
 // Fill the segment from the pre-sieve tiles (stored complemented: a set bit
 // marks a multiple of a tile prime): word w is the OR over the tiles of
 // window (B + 4w) mod P.  Tile 0 comes from global memory, the small
-// tiles from their LDS copy (kLds) or global memory.  Eight words per thread
-// per trip, so that all their loads are in flight before the first store.
+// tiles from their LDS copy (kLds) or global memory.  A thread fills runs of
+// 4 consecutive words: their windows are 4 adjacent tile words (4 loads that
+// merge into wide ones, one index update per run) and the run is one 16-byte
+// LDS store.  Two runs per thread per trip, so that all their loads are in
+// flight before the first store.
 template <bool kLds>
 __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint64_t B,
                                            const PSG_GLOBAL uint32_t* tiles, const uint32_t* small)
@@ -382,29 +385,36 @@ __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint
   uint32_t idx[PSG_NTILES], stp[PSG_NTILES];
 #pragma unroll
   for (int t = 0; t < PSG_NTILES; t++) {
-    idx[t] = (uint32_t) ((B + 4ull * tid) % kPeriod[t]);
-    stp[t] = (4u * nthreads) % kPeriod[t];
+    idx[t] = (uint32_t) ((B + 16ull * tid) % kPeriod[t]);
+    stp[t] = (16u * nthreads) % kPeriod[t];
   }
-  const auto word = [&](uint32_t w) {
-    uint32_t v = 0u;
+  const auto run = [&]() {
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
     for (int t = 0; t < PSG_NTILES; t++) {
       const uint32_t pos = (idx[t] & 3u) * kRows[t] + (idx[t] >> 2);
-      v |= (t == 0 || !kLds) ? tiles[kOffset[t] + pos] : small[kOffset[t] - kOffset[1] + pos];
+      if (t == 0 || !kLds) {
+        const PSG_GLOBAL uint32_t* q = tiles + kOffset[t] + pos;
+        v.x |= q[0]; v.y |= q[1]; v.z |= q[2]; v.w |= q[3];
+      } else {
+        const uint32_t* q = small + (kOffset[t] - kOffset[1]) + pos;
+        v.x |= q[0]; v.y |= q[1]; v.z |= q[2]; v.w |= q[3];
+      }
       idx[t] += stp[t];
       idx[t] = min(idx[t], idx[t] - kPeriod[t]);     // below 0 wraps to a large value
     }
     return v;
   };
-  uint32_t w = tid;
-  for (; w + 7u * nthreads < words; w += 8u * nthreads) {
-    uint32_t v[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = word(w + u * nthreads);
-#pragma unroll
-    for (int u = 0; u < 8; u++) sieve[w + u * nthreads] = v[u];
+  uint4* s4 = (uint4*) sieve;
+  const uint32_t runs = words >> 2;
+  uint32_t r = tid;
+  for (; r + nthreads < runs; r += 2u * nthreads) {
+    const uint4 a = run();
+    const uint4 b = run();
+    s4[r] = a;
+    s4[r + nthreads] = b;
   }
-  for (; w < words; w += nthreads) sieve[w] = word(w);
+  for (; r < runs; r += nthreads) s4[r] = run();
 }
 
 // Tier-3 primes per lane: a tier-3 item holds 64 * kT3 primes, so that the
