@@ -111,15 +111,27 @@ __device__ __forceinline__ uint32_t mod_big(uint64_t B, double Bf, uint32_t p, d
 }
 
 // Class table entry: ctab[(p % 30) * 8 + j] packs q_j (the cofactor residue
-// mod 30 of class j) in bits 0..23 and e_j + 1 in bits 24..31, where
-// c_j = (p/30)*q_j + e_j is the global byte of p*q_j.  A 24-bit multiply reads
-// only the low 24 bits, so c_j + 1 = mad24(p/30, entry, entry >> 24).
+// mod 30 of class j) in bits 0..15 and e_j + 1 in bits 16..31, where
+// c_j = (p/30)*q_j + e_j is the global byte of p*q_j, so
+// c_j + 1 = (p/30) * (entry & 0xffff) + (entry >> 16).
 //
 // (c - R) mod p for c, R < p: the unsigned difference wraps above 2^32 - p
 // when c < R, and then adding p brings it back below p.
 __device__ __forceinline__ uint32_t sub_mod(uint32_t c, uint32_t R, uint32_t p)
 {
   const uint32_t u = c - R;
+  return min(u, u + p);
+}
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// (c_j - R) mod p for p/30 < 2^16, from a1 = (p/30) | 1 << 16 and nR = ~R:
+// one unsigned 2-way dot product of the entry's halves with a1's,
+// q_j * (p/30) + (e_j + 1) * 1 + ~R = c_j - R (mod 2^32), then the wrap.
+__device__ __forceinline__ uint32_t class_offset(uint32_t ent, uint32_t a1, uint32_t nR, uint32_t p)
+{
+  const uint32_t u = __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, ent),
+                                            __builtin_bit_cast(u16x2, a1), nR, false);
   return min(u, u + p);
 }
 
@@ -135,7 +147,7 @@ __device__ __forceinline__ uint32_t class_start(uint32_t p, uint32_t R, uint64_t
   const uint32_t b = p - a * 30u;
   const uint32_t ent = ctab[b * 8u + j];
   const uint32_t q = ent & 0xffu;
-  uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R + 1u, p);   // a < 2^24 for every caller
+  uint32_t i = sub_mod(__umul24(a, ent & 0xffffu) + (ent >> 16), R + 1u, p);   // a < 2^24 for every caller
   if (kNear && q == 1u && B + i < (uint64_t) p) {
     const uint64_t ii = (uint64_t) i + p;
     i = (ii > 0xffffffffull) ? 0xffffffffu : (uint32_t) ii;
@@ -204,11 +216,11 @@ __device__ __forceinline__ void strike_medium(uint32_t p, uint32_t R, uint64_t B
 {
   const uint32_t a = p / 30u;
   const ClassRow row = class_row(ctab, p - a * 30u);
-  const uint32_t R1 = R + 1u;
+  const uint32_t a1 = a | 0x10000u, nR = ~R;                       // p <= S: a < 2^16
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
     const uint32_t ent = row.e[c];
-    uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);    // p <= S: a < 2^24
+    uint32_t i = class_offset(ent, a1, nR, p);
     if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
     strike_class(i, p, c, S);
@@ -224,12 +236,12 @@ __device__ __forceinline__ void strike_few(uint32_t p, uint32_t R, uint64_t B, u
 {
   const uint32_t a = p / 30u;
   const ClassRow row = class_row(ctab, p - a * 30u);
-  const uint32_t R1 = R + 1u;
+  const uint32_t a1 = a | 0x10000u, nR = ~R;                       // p <= S: a < 2^16
   const uint32_t rot = (p & 3u) << 3;
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
     const uint32_t ent = row.e[c];
-    uint32_t i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);
+    uint32_t i = class_offset(ent, a1, nR, p);
     if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
     uint32_t m = 1u << (((i & 3u) << 3) + c);
@@ -243,21 +255,20 @@ __device__ __forceinline__ void strike_few(uint32_t p, uint32_t R, uint64_t B, u
 
 // The <= 8 hits of a prime S < p in the segment at B (R = B mod p, a = p/30,
 // row = its class entries): one per residue class at most.  kSmall:
-// p < 30 * 2^24, so that (p/30) * q_j is a 24-bit multiply (and p < 2^31,
-// which sub_mod needs).
+// p < 30 * 2^16, so that class_offset applies.
 template <bool kNear, bool kSmall>
 __device__ __forceinline__ void strike_big(uint32_t p, uint32_t a, uint32_t R, uint64_t B,
                                            uint32_t S, const ClassRow& row)
 {
-  const uint32_t R1 = R + 1u;
+  const uint32_t a1 = a | 0x10000u, nR = ~R;
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
     const uint32_t ent = row.e[c];
     uint32_t i;
     if (kSmall) {
-      i = sub_mod(__umul24(a, ent) + (ent >> 24), R1, p);
+      i = class_offset(ent, a1, nR, p);
     } else {
-      const uint32_t cc = a * (ent & 0xffu) + (ent >> 24) - 1u;
+      const uint32_t cc = a * (ent & 0xffffu) + (ent >> 16) - 1u;
       i = (cc >= R) ? cc - R : cc - R + p;
     }
     if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
@@ -346,7 +357,7 @@ __device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRun& r)
 
 // Residue-class table, derived from first principles: for b = p mod 30 and
 // class j (offset o_j), q_j = o_j * b^-1 (mod 30) and e_j = (b*q_j - o_j)/30,
-// packed as q_j | (e_j + 1) << 24 (e_j is -1 only for o_j = 31, b = 1).
+// packed as q_j | (e_j + 1) << 16 (e_j is -1 only for o_j = 31, b = 1).
 __device__ void init_ctab(uint32_t* ctab)
 {
   for (uint32_t t = threadIdx.x; t < 240; t += blockDim.x) {
@@ -357,7 +368,7 @@ __device__ void init_ctab(uint32_t* ctab)
       for (uint32_t q = 1; q < 30; q++)
         if ((b * q) % 30u == o % 30u) {
           const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
-          ent = q | ((uint32_t) (e + 1) << 24);
+          ent = q | ((uint32_t) (e + 1) << 16);
           break;
         }
     }
@@ -627,7 +638,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
       const uint32_t u3 = u2 + (t3 - t2 + 64u * kT3 - 1u) / (64u * kT3);
       const uint32_t pTop = (t3 > t2) ? primes[t3 - 1u] : 0u;
       const bool near = B < (uint64_t) pTop;
-      const bool small = pTop < (30u << 24);
+      const bool small = pTop < (30u << 16);
       const uint64_t rmix = (((uint64_t) (u3 - u2) << 32) + u3 - 1u) / max(u3, 1u);
       const auto itemOf = [=](uint32_t t) -> uint32_t {
         if (t >= u3) return t;
