@@ -24,8 +24,8 @@
 //                tier 1 (256 < p <= 2048): lane = (prime, class), 8 primes
 //                        per work item; each class as four progressions of
 //                        stride 4p with loop-invariant masks;
-//                tier 2 (2048 < p <= S): lane = prime, 64 per item, the same
-//                        progressions for each of its classes;
+//                tier 2 (2048 < p <= S): 2 primes per lane, 128 per item,
+//                        the same progressions for each of their classes;
 //                tier 3 (S < p <= bucket threshold): 4 primes per lane, 256
 //                        per item, at most one hit per class;
 //              the items of tiers 1-3 form one queue per segment that the
@@ -208,14 +208,13 @@ __device__ __forceinline__ void strike_class(uint32_t i, uint32_t p, uint32_t c,
   if (q2 < end) lds_or(q2, m2);
 }
 
-// All hits of a prime p <= S in the segment at B (R = B mod p): its 8
-// residue classes, each an arithmetic progression of bytes with stride p.
+// All hits of a prime p <= S in the segment at B (R = B mod p, a = p/30,
+// row = its class entries): its 8 residue classes, each an arithmetic
+// progression of bytes with stride p.
 template <bool kNear>
-__device__ __forceinline__ void strike_medium(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
-                                              const uint32_t* ctab)
+__device__ __forceinline__ void strike_medium(uint32_t p, uint32_t a, uint32_t R, uint64_t B,
+                                              uint32_t S, const ClassRow& row)
 {
-  const uint32_t a = p / 30u;
-  const ClassRow row = class_row(ctab, p - a * 30u);
   const uint32_t a1 = a | 0x10000u, nR = ~R;                       // p <= S: a < 2^16
 #pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
@@ -231,11 +230,9 @@ __device__ __forceinline__ void strike_medium(uint32_t p, uint32_t R, uint64_t B
 // (nh is uniform over the wave): the hits of a class one after another, the
 // word mask rotated by 8 * (p mod 4) bits from one hit to the next.
 template <bool kNear>
-__device__ __forceinline__ void strike_few(uint32_t p, uint32_t R, uint64_t B, uint32_t S,
-                                           const uint32_t* ctab, uint32_t nh)
+__device__ __forceinline__ void strike_few(uint32_t p, uint32_t a, uint32_t R, uint64_t B,
+                                           uint32_t S, const ClassRow& row, uint32_t nh)
 {
-  const uint32_t a = p / 30u;
-  const ClassRow row = class_row(ctab, p - a * 30u);
   const uint32_t a1 = a | 0x10000u, nR = ~R;                       // p <= S: a < 2^16
   const uint32_t rot = (p & 3u) << 3;
 #pragma unroll
@@ -250,6 +247,22 @@ __device__ __forceinline__ void strike_few(uint32_t p, uint32_t R, uint64_t B, u
       i += p;
       m = rotl32(m, rot);
     }
+  }
+}
+
+// A tier-2 prime: the few-hit path when the smallest prime of its half of the
+// item (pMin, uniform) is above S/4, else the progressions.
+__device__ __forceinline__ void strike_tier2(uint32_t p, uint32_t a, uint32_t R, uint64_t B,
+                                             uint32_t S, const ClassRow& row, uint32_t pMin,
+                                             bool nearSmall)
+{
+  if (4u * pMin > S) {
+    const uint32_t nh = (2u * pMin > S) ? 2u : (3u * pMin > S) ? 3u : 4u;
+    if (nearSmall) strike_few<true>(p, a, R, B, S, row, nh);
+    else strike_few<false>(p, a, R, B, S, row, nh);
+  } else {
+    if (nearSmall) strike_medium<true>(p, a, R, B, S, row);
+    else strike_medium<false>(p, a, R, B, S, row);
   }
 }
 
@@ -444,9 +457,9 @@ static_assert((kCtabWords + kVarWords + 3 * kTier0Slots) * 4 == PSG_SIEVE_LDS_TA
 
 // First prime index of lane `lane` in work item `it` of a segment's queue
 // (see sieve_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each
-// (lane = prime * 8 + class), [u1, u2) 64 primes of [t1, t2) (lane = prime),
-// [u2, u3) 64 * kT3 primes of [t2, t3) (lane m-th prime: index + 64 m).  `it`
-// is uniform.
+// (lane = prime * 8 + class), [u1, u2) 128 primes of [t1, t2) (2 per lane:
+// index, index + 64), [u2, u3) 64 * kT3 primes of [t2, t3) (lane m-th prime:
+// index + 64 m).  `it` is uniform.
 __device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint32_t t0, uint32_t t1,
                                                uint32_t t2, uint32_t t3, uint32_t u1, uint32_t u2,
                                                uint32_t u3)
@@ -456,7 +469,7 @@ __device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint3
     k = t0 + (it << 3) + (lane >> 3);
     end = t1;
   } else if (it < u2) {
-    k = t1 + ((it - u1) << 6) + lane;
+    k = t1 + (it - u1) * 128u + lane;
     end = t2;
   } else {
     k = t2 + (it - u2) * (64u * kT3) + lane;
@@ -617,7 +630,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     //   tier 1 (256 < p <= 2048): 8 primes per item, lane = (prime, class),
     //     each lane strikes one class over the whole segment (the primes of
     //     an item are close, so its lanes finish together);
-    //   tier 2 (2048 < p <= S): 64 primes per item, lane = prime;
+    //   tier 2 (2048 < p <= S): 128 primes per item, 2 per lane;
     //   tier 3 (S < p <= bucket threshold): 64 * kT3 primes per item, kT3 per
     //     lane, <= 1 hit per class; the prime itself can only be in the
     //     segment while B < p (uniform test against the largest tier-3 prime).
@@ -634,7 +647,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     // is struck at the start of the item after the one it was loaded in.
     {
       const uint32_t u1 = (t1 - t0 + 7u) >> 3;
-      const uint32_t u2 = u1 + ((t2 - t1 + 63u) >> 6);
+      const uint32_t u2 = u1 + (t2 - t1 + 127u) / 128u;
       const uint32_t u3 = u2 + (t3 - t2 + 64u * kT3 - 1u) / (64u * kT3);
       const uint32_t pTop = (t3 > t2) ? primes[t3 - 1u] : 0u;
       const bool near = B < (uint64_t) pTop;
@@ -660,7 +673,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
 #pragma unroll
       for (uint32_t m = 0; m < kT3 - 1; m++) {
         const uint32_t km = k + 64u * (m + 1u);
-        const bool ok = cur >= u2 && k != 0xffffffffu && km < t3;
+        const bool ok = k != 0xffffffffu && (cur >= u2 ? km < t3 : (cur >= u1 && m == 0 && km < t2));
         pX[m] = ok ? primes[km] : 0u;
         MX[m] = ok ? magics[km] : 0ull;
       }
@@ -697,7 +710,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
 #pragma unroll
         for (uint32_t m = 0; m < kT3 - 1; m++) {
           const uint32_t km = kN + 64u * (m + 1u);
-          const bool ok = nxt >= u2 && kN != 0xffffffffu && km < t3;
+          const bool ok = kN != 0xffffffffu && (nxt >= u2 ? km < t3 : (nxt >= u1 && m == 0 && km < t2));
           pNX[m] = ok ? primes[km] : 0u;
           MNX[m] = ok ? magics[km] : 0ull;
         }
@@ -708,16 +721,16 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
                                          : class_start<false>(p, R, B, j, ctab);
             strike_class(i, p, j, S);
           } else if (cur < u2) {
-            // lane 0 holds the item's smallest prime
-            const uint32_t pMin = uniform(p);
-            if (4u * pMin > S) {
-              const uint32_t nh = (2u * pMin > S) ? 2u : (3u * pMin > S) ? 3u : 4u;
-              if (nearSmall) strike_few<true>(p, R, B, S, ctab, nh);
-              else strike_few<false>(p, R, B, S, ctab, nh);
-            } else {
-              if (nearSmall) strike_medium<true>(p, R, B, S, ctab);
-              else strike_medium<false>(p, R, B, S, ctab);
-            }
+            // two primes per lane (k, k + 64), both class rows read before
+            // the first strike; lane 0 holds each half's smallest prime
+            const uint32_t p2 = pX[0];
+            const uint32_t R2 = p2 ? mod_magic(B, p2, MX[0]) : 0u;
+            const uint32_t pMin = uniform(p), pMin2 = uniform(p2);
+            const uint32_t a = p / 30u, a2 = p2 / 30u;
+            const ClassRow row = class_row(ctab, p - a * 30u);
+            const ClassRow row2 = class_row(ctab, p2 - a2 * 30u);
+            strike_tier2(p, a, R, B, S, row, pMin, nearSmall);
+            if (p2) strike_tier2(p2, a2, R2, B, S, row2, pMin2, nearSmall);
           } else {
             uint32_t pa[kT3], Ra[kT3];
             pa[0] = p;
