@@ -564,7 +564,7 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
   // that their HBM latency passes during tier 0, which waits on no global
   // load (after the fill: its tile loads would otherwise wait for them).
   // They are struck when the queue starts.
-  constexpr uint32_t kBucketPre = 16;
+  constexpr uint32_t kBucketPre = 8;
   uint32_t bpre[kBucketPre];
   uint32_t bE = 0, bE1 = 0;
   if (args.buckets) {
