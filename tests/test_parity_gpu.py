@@ -167,6 +167,20 @@ def test_bucket_list_room_retry(gpu, oracle):
         gpu.set_tuning()
 
 
+def test_wide_tier3_against_oracle(gpu, oracle):
+    """A bucket threshold of 24 segments puts tier-3 primes above 30 * 2^16,
+    where the class starts take the 32-bit multiply path instead of the packed
+    16-bit dot product; the counts must not change."""
+    gpu.set_tuning(bucket_factor=24)
+    try:
+        assert gpu.get_tuning()["bucket_factor"] == 24
+        for a, b in [(10 ** 14, 10 ** 14 + 10 ** 7), (10 ** 15 + 12345, 10 ** 15 + 3 * 10 ** 6)]:
+            _say("bucket factor 24:", a, b)
+            assert gpu.count_all(a, b) == oracle.count6(a, b), (a, b)
+    finally:
+        gpu.set_tuning()
+
+
 def test_measured_ceilings(gpu):
     """The roofline microbenchmarks the benchmark divides by: every figure is
     positive, the conflict-free LDS atomics are at least as fast as random
