@@ -664,7 +664,8 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
       const auto uniform = [](uint32_t v) { return (uint32_t) __builtin_amdgcn_readfirstlane((int) v); };
       uint32_t cur = itemOf(uniform(ticket()));
       uint32_t nxt = itemOf(uniform(ticket()));
-      // the lane's first prime of the current item, and its further tier-3 primes
+      // the lane's first prime of the current item and its further ones (k + 64 m:
+      // one more in tier 2, kT3 - 1 more in tier 3)
       uint32_t k = primeOf(cur);
       uint32_t p = (k != 0xffffffffu) ? primes[k] : 0u;
       uint64_t M = (k != 0xffffffffu) ? magics[k] : 0ull;
