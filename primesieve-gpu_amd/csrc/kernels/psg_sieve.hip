@@ -35,9 +35,11 @@
 //              psg_bucket_fill for larger primes (one LDS atomic per entry),
 //              whose entries are loaded while the queue's items are struck;
 //   3. count:  popcount and byte-lane SWAR masks give all six counts in one
-//              pass; wave sums use ds_swizzle butterflies.  In list mode the
-//              segment's primes are written out instead (the device's
-//              sieving-prime cache; psg_prime_magics adds their magics).
+//              pass; wave sums use ds_swizzle butterflies.  The same pass
+//              writes the next segment's fill (step 1) into each chunk it
+//              has counted.  In list mode the segment's primes are written
+//              out instead (the device's sieving-prime cache;
+//              psg_prime_magics adds their magics).
 //
 // In LDS a set bit marks a composite (the counts read the complement), so a
 // strike is a ds_or_b32 atomic with a one-bit mask: several lanes may hit
@@ -371,7 +373,7 @@ __device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRun& r)
 // Residue-class table, derived from first principles: for b = p mod 30 and
 // class j (offset o_j), q_j = o_j * b^-1 (mod 30) and e_j = (b*q_j - o_j)/30,
 // packed as q_j | (e_j + 1) << 16 (e_j is -1 only for o_j = 31, b = 1).
-__device__ void init_ctab(uint32_t* ctab)
+__device__ __forceinline__ void init_ctab(uint32_t* ctab)
 {
   for (uint32_t t = threadIdx.x; t < 240; t += blockDim.x) {
     const uint32_t b = t >> 3, j = t & 7u;
@@ -456,7 +458,7 @@ static_assert((kCtabWords + kVarWords + 3 * kTier0Slots) * 4 == PSG_SIEVE_LDS_TA
               "LDS tail layout");
 
 // First prime index of lane `lane` in work item `it` of a segment's queue
-// (see sieve_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each
+// (see strike_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each
 // (lane = prime * 8 + class), [u1, u2) 128 primes of [t1, t2) (2 per lane:
 // index, index + 64), [u2, u3) 64 * kT3 primes of [t2, t3) (lane m-th prime:
 // index + 64 m).  `it` is uniform.
@@ -483,7 +485,7 @@ __device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint3
 // an exclusive scan of the per-thread counts (waves by shuffles, the wave
 // totals through LDS) gives each thread its first output index.  The run is
 // one segment, so its output starts at outOffsets[run.out].
-__device__ void list_primes(const PsgSieveArgs& args, const PsgRun& run, uint64_t B,
+__device__ __forceinline__ void list_primes(const PsgSieveArgs& args, const PsgRun& run, uint64_t B,
                             const uint32_t* sieve, uint32_t* waveTot)
 {
   const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
@@ -523,47 +525,65 @@ __device__ void list_primes(const PsgSieveArgs& args, const PsgRun& run, uint64_
   }
 }
 
-// One segment: fill, strike, count.  Ends with a barrier, so that the LDS
-// sieve and the wave sums may be reused by the next segment.
-__device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const PsgRun& run,
-                                              uint32_t seg, uint32_t* sieve,
-                                              const uint32_t* ctab, uint32_t* wsum,
-                                              const uint32_t* small)
+// The tiles also mark the tile primes 7..47 themselves: all of global byte 0
+// (7..31) and bits 0-3 of global byte 1 (37..47).  A segment that starts at
+// byte 0 or 1 has them in its word 0, which thread 0 wrote and restores.
+__device__ __forceinline__ void restore_tile_primes(uint32_t* sieve, uint64_t B)
+{
+  if (B < 2u && threadIdx.x == 0)
+    sieve[0] &= (B == 0) ? ~0x0fffu : ~0x000fu;
+}
+
+// A run read from global memory at a uniform index, its fields made
+// uniform (scalar registers) so that they do not occupy vector registers.
+__device__ __forceinline__ PsgRun load_run(const PSG_GLOBAL PsgRun* runs, uint32_t i)
+{
+  union { PsgRun r; uint32_t w[sizeof(PsgRun) / 4]; } u;
+  u.r = runs[i];
+#pragma unroll
+  for (uint32_t k = 0; k < sizeof(PsgRun) / 4; k++)
+    u.w[k] = (uint32_t) __builtin_amdgcn_readfirstlane((int) u.w[k]);
+  return u.r;
+}
+
+// Step 1 for the segment at B: the pre-sieve fill (or zeros without tiles).
+// No barrier.
+__device__ __forceinline__ void fill_segment(const PsgSieveArgs& args, uint64_t B, uint32_t* sieve,
+                                             const uint32_t* small)
+{
+  const uint32_t tid = threadIdx.x;
+  const uint32_t words = args.segBytes >> 2;
+  const auto* __restrict__ tiles = gptr<uint32_t>(args.tiles);
+  if (!tiles) {
+    // pre-sieving disabled: every prime >= 7 is struck in step 2
+    for (uint32_t w = tid; w < words; w += blockDim.x)
+      sieve[w] = 0u;
+  } else {
+    if (args.tilesInLds) fill_tiles<true>(sieve, words, B, tiles, small);
+    else fill_tiles<false>(sieve, words, B, tiles, small);
+    restore_tile_primes(sieve, B);
+  }
+}
+
+// Step 2 for segment `seg` at B of `run`: strike every sieving prime and the
+// segment's bucket list.  The queue counter must be 0; no barrier.
+__device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const PsgRun& run,
+                                               uint32_t seg, uint64_t B, uint32_t* sieve,
+                                               const uint32_t* ctab, uint32_t* wsum)
 {
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
   const uint32_t lane = tid & 63u;
   const uint32_t S = args.segBytes;
-  const uint32_t words = S >> 2;
-  const uint64_t B = run.B0 + (uint64_t) (seg - run.segStart) * S;
 
   const auto* __restrict__ primes = gptr<uint32_t>(args.primes);
   const auto* __restrict__ magics = gptr<uint64_t>(args.magics);
-  const auto* __restrict__ tiles = gptr<uint32_t>(args.tiles);
-  uint32_t* sWork = wsum + PSG_NCOUNTS + 2;        // work queue of step 2, reset below
-  if (tid == 0) *sWork = 0u;
+  uint32_t* sWork = wsum + PSG_NCOUNTS + 2;        // the work queue's counter
 
-  // ---- 1. fill from the pre-sieve tiles ----------------------------------
-  if (!tiles) {
-    // pre-sieving disabled: every prime >= 7 is struck in step 2
-    for (uint32_t w = tid; w < words; w += nthreads)
-      sieve[w] = 0u;
-  } else {
-    if (args.tilesInLds) fill_tiles<true>(sieve, words, B, tiles, small);
-    else fill_tiles<false>(sieve, words, B, tiles, small);
-    // the tiles also marked the tile primes 7..47 themselves: all of global
-    // byte 0 (7..31) and bits 0-3 of global byte 1 (37..47).  A segment
-    // that starts at byte 0 or 1 has them in its word 0, which thread 0
-    // wrote above and restores without a barrier.
-    if (B < 2u && tid == 0)
-      sieve[0] &= (B == 0) ? ~0x0fffu : ~0x000fu;
-  }
-  __syncthreads();
-
-  // The first kBucketPre bucket entries of every thread are loaded now, so
-  // that their HBM latency passes during tier 0, which waits on no global
-  // load (after the fill: its tile loads would otherwise wait for them).
-  // They are struck when the queue starts.
+  // The first kBucketPre bucket entries of every thread are loaded now (after
+  // the fill: its tile loads would otherwise wait for them), so that their
+  // HBM latency passes during tier 0, which waits on no global load.  They
+  // are struck when the queue starts.
   constexpr uint32_t kBucketPre = 8;
   uint32_t bpre[kBucketPre];
   uint32_t bE = 0, bE1 = 0;
@@ -782,61 +802,100 @@ __device__ __forceinline__ void sieve_segment(const PsgSieveArgs& args, const Ps
     }
     for (; e < e1; e += nthreads) strike_bit(ent[e]);
   }
-  __syncthreads();
+}
 
-  // ---- 3. count all six types (list mode: write the primes) --------------
-  if (args.primesOut) {
-    list_primes(args, run, B, sieve, wsum + PSG_NCOUNTS + 3);
-    __syncthreads();
-    return;
+// Step 3 for the segment at B of `run`, fused with step 1 of the next one
+// (at nB, when hasNext): one pass over the LDS sieve in 16-byte chunks, each
+// read, counted (all six types) and overwritten with the next segment's
+// pre-sieve fill by the same thread.  The count's VALU work then covers the
+// fill's tile loads.  Adds this thread's counts to c[].  No barrier.
+__device__ __forceinline__ void count_and_fill(const PsgSieveArgs& args, const PsgRun& run,
+                                               uint64_t B, bool hasNext, uint64_t nB,
+                                               uint32_t* sieve, const uint32_t* small,
+                                               uint32_t (&c)[PSG_NCOUNTS])
+{
+  constexpr uint32_t kPeriod[PSG_NTILES] = PSG_TILE_PERIODS;
+  constexpr uint32_t kRows[PSG_NTILES] = PSG_TILE_ROWS;
+  constexpr uint32_t kOffset[PSG_NTILES] = PSG_TILE_OFFSETS;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthreads = blockDim.x;
+  const uint32_t S = args.segBytes;
+  const auto* __restrict__ tiles = gptr<uint32_t>(args.tiles);
+  const bool fill = hasNext && tiles;              // else the next segment starts from zeros
+  const bool lds = args.tilesInLds != 0;
+  uint32_t idx[PSG_NTILES], stp[PSG_NTILES];
+#pragma unroll
+  for (int t = 0; t < PSG_NTILES; t++) {
+    idx[t] = fill ? (uint32_t) ((nB + 16ull * tid) % kPeriod[t]) : 0u;
+    stp[t] = (16u * nthreads) % kPeriod[t];
   }
-  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
-  {
-    const uint64_t* s64 = (const uint64_t*) sieve;
-    const uint32_t words64 = S >> 3;
-    const auto add = [&](uint64_t x) {
-      const uint64_t t1 = x & (x >> 1);
-      const uint64_t t2 = t1 & (x >> 2);
-      const uint64_t t3 = t2 & (x >> 3);
-      const uint64_t t4 = t3 & (x >> 4);
-      const uint64_t t5 = t4 & (x >> 5);
-      c0 += __popcll(x);
-      c1 += __popcll(t1 & 0x4a4a4a4a4a4a4a4aull);   // (11,13) (17,19) (29,31)
-      c2 += __popcll(t2 & 0x0f0f0f0f0f0f0f0full);   // 3 consecutive bits from 0..3
-      c3 += __popcll(t3 & 0x0202020202020202ull);   // bits 1..4
-      c4 += __popcll(t4 & 0x0303030303030303ull);   // bits 0..4, 1..5
-      c5 += __popcll(t5 & 0x0101010101010101ull);   // bits 0..5
-    };
-    uint32_t w = tid;
-    if (B > run.gLo && B + S <= run.gHi) {
-      // the whole segment lies inside the interval: no edge masks, and four
-      // LDS reads in flight per thread
-      for (; w + 3u * nthreads < words64; w += 4u * nthreads) {
-        uint64_t x[4];
+  const auto next_run = [&]() {                    // as in fill_tiles
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-        for (uint32_t u = 0; u < 4; u++) x[u] = ~s64[w + u * nthreads];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++) add(x[u]);
+    for (int t = 0; t < PSG_NTILES; t++) {
+      const uint32_t pos = (idx[t] & 3u) * kRows[t] + (idx[t] >> 2);
+      if (t == 0 || !lds) {
+        const PSG_GLOBAL uint32_t* q = tiles + kOffset[t] + pos;
+        v.x |= q[0]; v.y |= q[1]; v.z |= q[2]; v.w |= q[3];
+      } else {
+        const uint32_t* q = small + (kOffset[t] - kOffset[1]) + pos;
+        v.x |= q[0]; v.y |= q[1]; v.z |= q[2]; v.w |= q[3];
       }
+      idx[t] += stp[t];
+      idx[t] = min(idx[t], idx[t] - kPeriod[t]);     // below 0 wraps to a large value
     }
-    for (; w < words64; w += nthreads) {
-      uint64_t x = ~s64[w];
-      const uint64_t gw = B + 8ull * w;
-      if (gw <= run.gLo || gw + 7 >= run.gHi)
-        x = mask_edges(x, gw, run);
-      add(x);
+    return v;
+  };
+  const auto add = [&](uint64_t x) {
+    const uint64_t t1 = x & (x >> 1);
+    const uint64_t t2 = t1 & (x >> 2);
+    const uint64_t t3 = t2 & (x >> 3);
+    const uint64_t t4 = t3 & (x >> 4);
+    const uint64_t t5 = t4 & (x >> 5);
+    c[0] += __popcll(x);
+    c[1] += __popcll(t1 & 0x4a4a4a4a4a4a4a4aull);   // (11,13) (17,19) (29,31)
+    c[2] += __popcll(t2 & 0x0f0f0f0f0f0f0f0full);   // 3 consecutive bits from 0..3
+    c[3] += __popcll(t3 & 0x0202020202020202ull);   // bits 1..4
+    c[4] += __popcll(t4 & 0x0303030303030303ull);   // bits 0..4, 1..5
+    c[5] += __popcll(t5 & 0x0101010101010101ull);   // bits 0..5
+  };
+  // the whole segment inside the interval: no edge masks
+  const bool inside = B > run.gLo && B + S <= run.gHi;
+  const auto count = [&](const uint4& v, uint32_t r) {   // chunk r: 64-bit words 2r, 2r+1
+    uint64_t x0 = ~((uint64_t) v.x | ((uint64_t) v.y << 32));
+    uint64_t x1 = ~((uint64_t) v.z | ((uint64_t) v.w << 32));
+    if (!inside) {
+      const uint64_t g0 = B + 16ull * r, g1 = g0 + 8u;
+      if (g0 <= run.gLo || g0 + 7 >= run.gHi) x0 = mask_edges(x0, g0, run);
+      if (g1 <= run.gLo || g1 + 7 >= run.gHi) x1 = mask_edges(x1, g1, run);
+    }
+    add(x0);
+    add(x1);
+  };
+  uint4* s4 = (uint4*) sieve;
+  const uint32_t chunks = S >> 4;
+  uint32_t r = tid;
+  for (; r + nthreads < chunks; r += 2u * nthreads) {
+    const uint4 a = s4[r], b = s4[r + nthreads];
+    uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = fa;
+    if (fill) {
+      fa = next_run();
+      fb = next_run();
+    }
+    count(a, r);
+    count(b, r + nthreads);
+    if (hasNext) {
+      s4[r] = fa;
+      s4[r + nthreads] = fb;
     }
   }
-  c0 = wave_sum(c0); c1 = wave_sum(c1); c2 = wave_sum(c2);
-  c3 = wave_sum(c3); c4 = wave_sum(c4); c5 = wave_sum(c5);
-  if (lane == 0) {
-    atomicAdd(&wsum[0], c0); atomicAdd(&wsum[1], c1); atomicAdd(&wsum[2], c2);
-    atomicAdd(&wsum[3], c3); atomicAdd(&wsum[4], c4); atomicAdd(&wsum[5], c5);
+  for (; r < chunks; r += nthreads) {
+    const uint4 a = s4[r];
+    const uint4 fa = fill ? next_run() : make_uint4(0u, 0u, 0u, 0u);
+    count(a, r);
+    if (hasNext) s4[r] = fa;
   }
-  __syncthreads();
-  if (tid < PSG_NCOUNTS && wsum[tid])
-    atomicAdd((unsigned long long*) args.counts + (uint64_t) run.out * PSG_NCOUNTS + tid,
-              (unsigned long long) wsum[tid]);
+  if (fill) restore_tile_primes(sieve, nB);
 }
 
 // Last index i in [0, n) with key(i) <= x (keys ascending, key(0) <= x).
@@ -1008,19 +1067,72 @@ psg_sieve_count(PsgSieveArgs args)
       t0m[i] = (i < n0) ? mg[i] : 0ull;
     }
   }
+  // Per segment: [strike] barrier [count this one + fill the next] barrier
+  // [publish the counts].  The first segment is filled on its own.
+  uint32_t* sWork = wsum + PSG_NCOUNTS + 2;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const auto take = [&]() {                        // thread 0: the next segment and its run
+    const uint32_t seg = atomicAdd((uint32_t*) args.workCounter, 1u);
+    sSeg = seg;
+    sRun = (seg < args.nSegs)
+         ? last_le(args.nRuns, seg, [&](uint32_t i) { return runs[i].segStart; }) : 0u;
+  };
+  if (tid == 0) {
+    take();
+    *sWork = 0u;
+  }
+  if (tid < PSG_NCOUNTS) wsum[tid] = 0;
+  __syncthreads();
+  uint32_t seg = sSeg;
+  if (seg >= args.nSegs) return;
+  PsgRun run = load_run(runs, sRun);
+  uint64_t B = run.B0 + (uint64_t) (seg - run.segStart) * args.segBytes;
+  bool filled = false;                             // by the previous segment's count pass
   for (;;) {
-    if (threadIdx.x == 0) {
-      const uint32_t seg = atomicAdd((uint32_t*) args.workCounter, 1u);
-      sSeg = seg;
-      sRun = (seg < args.nSegs)
-           ? last_le(args.nRuns, seg, [&](uint32_t i) { return runs[i].segStart; }) : 0u;
+    if (!filled) {
+      fill_segment(args, B, sieve, small);
+      __syncthreads();
     }
-    if (threadIdx.x < PSG_NCOUNTS) wsum[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t seg = sSeg;
-    if (seg >= args.nSegs) break;
-    const PsgRun run = runs[sRun];
-    sieve_segment(args, run, seg, sieve, ctab, wsum, small);
+    strike_segment(args, run, seg, B, sieve, ctab, wsum);
+    if (tid == 0) take();
+    __syncthreads();                               // strikes done, next segment known
+    const uint32_t next = sSeg;
+    const bool hasNext = next < args.nSegs;
+    PsgRun nrun = run;
+    uint64_t nB = 0;
+    if (hasNext) {
+      nrun = load_run(runs, sRun);
+      nB = nrun.B0 + (uint64_t) (next - nrun.segStart) * args.segBytes;
+    }
+    if (tid == 0) *sWork = 0u;                     // every wave has left the queue
+    if (args.primesOut) {
+      // list mode: write the primes; the next segment is filled at the top
+      list_primes(args, run, B, sieve, wsum + PSG_NCOUNTS + 3);
+      __syncthreads();
+      filled = false;
+    } else {
+      uint32_t c[PSG_NCOUNTS] = {0u, 0u, 0u, 0u, 0u, 0u};
+      count_and_fill(args, run, B, hasNext, nB, sieve, small, c);
+#pragma unroll
+      for (int t = 0; t < PSG_NCOUNTS; t++) c[t] = wave_sum(c[t]);
+      if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < PSG_NCOUNTS; t++) atomicAdd(&wsum[t], c[t]);
+      }
+      __syncthreads();                             // counts summed, next segment filled
+      if (tid < PSG_NCOUNTS) {
+        if (wsum[tid])
+          atomicAdd((unsigned long long*) args.counts + (uint64_t) run.out * PSG_NCOUNTS + tid,
+                    (unsigned long long) wsum[tid]);
+        wsum[tid] = 0;                             // summed again after the next strikes
+      }
+      filled = true;
+    }
+    if (!hasNext) break;
+    seg = next;
+    run = nrun;
+    B = nB;
   }
 }
 
