@@ -22,8 +22,9 @@ Prints ONE JSON line:
                 traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE /
                 WRITE_SIZE summary in profiles/ when present
   cpu_baseline  the reference CPU sieve (oracle/_ref, built from the
-                reference sources) on the host cores, counting primes over the
-                same interval
+                reference sources) on the host's CPU share (16 threads),
+                counting all six types over the same interval in one pass;
+                the host's core count, quota and CPU model are recorded
   workloads     the other configurations: pi(1e10); 8192 uneven intervals
                 batched vs one call per interval (intervals/s), with and
                 without the cost model; the cost model's training and regret;
@@ -53,26 +54,58 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """What the host offers: logical CPUs, the CPUs this process may run on,
+    the cgroup CPU quota and the CPU model."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        q = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q and q[0] != "max":
+            info["cgroup_cpu_quota"] = float(q[0]) / float(q[1])
+    except (OSError, ValueError, IndexError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
+
+
+# Worker threads of the CPU baseline: the host's CPU share for one GPU on
+# these machines is 16 cores, whatever os.cpu_count() reports.
+CPU_SHARE = 16
+
+
 def cpu_baseline():
-    """The reference CPU library on the host's cores (counting primes over
-    the whole interval; the CPU's sieving work is the same for six counts)."""
+    """The reference CPU library (oracle/_ref, built from the reference
+    sources) counting all six types over the whole interval in ONE pass --
+    its ParallelSieve with the six COUNT_* flags, as `primesieve 1e13
+    --dist=1e11 --count=123456` runs -- on the host's CPU share."""
     ref = os.path.join(ROOT, "oracle", "_ref", "libprimesieve_ref.so")
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    threads = max(1, min(CPU_SHARE, host.get("affinity_cpus") or host["os_cpu_count"] or 1))
     if os.path.exists(ref):
         L = ctypes.CDLL(ref)
-        L.primesieve_count_primes.restype = ctypes.c_uint64
-        L.primesieve_count_primes.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
-        L.primesieve_set_num_threads.argtypes = [ctypes.c_int]
-        L.primesieve_set_num_threads(threads)
+        L.psg_ref_count6.restype = None
+        L.psg_ref_count6.argtypes = [ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
+        out = (ctypes.c_uint64 * 6)()
         t0 = time.perf_counter()
-        n = L.primesieve_count_primes(START, STOP)
+        L.psg_ref_count6(START, STOP, out, threads)
         dt = time.perf_counter() - t0
-        if n != GOLDEN[0]:
-            raise RuntimeError("reference CPU count mismatch: %d" % n)
+        if list(out) != GOLDEN:
+            raise RuntimeError("reference CPU counts mismatch: %s" % list(out))
         return {"value": (STOP - START + 1) / dt, "unit": "integers/s", "cores": threads,
-                "kind": "reference",
-                "sample": "primesieve_count_primes over [1e13, 1e13+1e11] (the whole interval), "
-                          "%d threads, %.2f s" % (threads, dt)}
+                "kind": "reference", "host": host, "seconds": dt,
+                "sample": "reference ParallelSieve, all six counts of [1e13, 1e13+1e11] (the whole "
+                          "interval) in one pass, %d threads, %.2f s" % (threads, dt)}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from conftest import Oracle
     a, b = START, START + 10 ** 10
@@ -80,6 +113,7 @@ def cpu_baseline():
     Oracle().count6(a, b, threads)
     dt = time.perf_counter() - t0
     return {"value": (b - a + 1) / dt, "unit": "integers/s", "cores": threads, "kind": "port",
+            "host": host, "seconds": dt,
             "sample": "oracle count6 over [1e13, 1e13+1e10], %d threads, %.2f s" % (threads, dt)}
 
 
@@ -124,11 +158,20 @@ def workload_pi(m):
 def workload_batch(m, model):
     q = np.load(os.path.join(ROOT, "tests", "golden", "config3_queries.npy"), allow_pickle=False)
     qs = [(int(a), int(b), int(k)) for a, b, k in q]
-    m.count_all(10 ** 18, 10 ** 18 + 10 ** 6)      # sieving primes up to 1e9 cached (untimed)
     res = {"workload": "8192 uneven intervals, start 1e9..1e18, length 2^16..2^32, k 1..6"}
+    # cold: a fresh backend builds its sieving-prime cache (to ~1e9, on the
+    # GPU) inside the timed call
+    m.set_tuning()
+    t0 = time.perf_counter()
+    cold = m.count_batch(qs)
+    res["cold_batched_s"] = time.perf_counter() - t0
+    res["cold_batched_intervals_per_s"] = len(qs) / res["cold_batched_s"]
+    # warm: the cache stays resident between calls
     t0 = time.perf_counter()
     batched = m.count_batch(qs)
     res["batched_s"] = time.perf_counter() - t0
+    if cold != batched:
+        raise SystemExit("cold and warm batched counts differ")
     res["batched_stats"] = {k: v for k, v in m.last_stats().as_dict().items()
                             if k in ("kernel_ms", "fill_ms", "device_ms", "launches", "segments")}
     t0 = time.perf_counter()
@@ -137,6 +180,7 @@ def workload_batch(m, model):
     if plain != batched:
         raise SystemExit("batched and plain counts differ")
     exp = json.load(open(os.path.join(ROOT, "tests", "golden", "config3_expected.json")))
+    exp.update(json.load(open(os.path.join(ROOT, "tests", "golden", "config3_decades.json"))))
     bad = [i for i, v in exp.items() if batched[int(i)] != v]
     if bad:
         raise SystemExit("batched counts differ from the reference on %d queries" % len(bad))
@@ -200,9 +244,15 @@ def workload_serving(m, model, seconds=4.0, qps=2000.0):
     check = m.count_batch(qs, mode=m.BATCH_PLAIN)
     if check != counts:
         raise SystemExit("served counts differ from the plain path")
+    exp = json.load(open(os.path.join(ROOT, "tests", "golden", "config5_expected.json")))
+    exp = exp.get("5:%d" % n, {})
+    bad = [i for i, v in exp.items() if counts[int(i)] != v]
+    if bad:
+        raise SystemExit("served counts differ from the reference on %d queries" % len(bad))
     s = serve.summary(lat, wall, n, batches)
     s["workload"] = "Poisson %.0f QPS for %.0f s, Pareto(1.2) lengths in [2^12, 2^36], k 1..6" % (qps, seconds)
-    s["counts_checked"] = "all %d against the plain per-query path" % n
+    s["counts_checked"] = ("all %d against the plain per-query path, %d against the reference CPU library"
+                           % (n, len(exp)))
     return s
 
 

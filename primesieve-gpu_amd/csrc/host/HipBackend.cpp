@@ -96,15 +96,24 @@ void HipBackend::init()
   while (sb > 256u && (uint64_t) sb + PSG_SIEVE_LDS_TAIL > ldsMax) sb -= 256u;  // the segment and its tail
   sieveBytes_ = sb;
 
+  // A power of two, as the reference requires of workGroupSize
+  // (src/gpu/GpuBackend.hpp:49-50, rounded down in CudaBackend.cpp:260-263):
+  // tier 0 splits a segment into equal per-wave slices of S / waves bytes,
+  // which must be whole 16-byte multiples for every S that is a multiple of
+  // 256, i.e. waves must divide 16.
   uint32_t wg = config_.workGroupSize ? config_.workGroupSize : 1024u;
-  wg = (wg / 64u) * 64u;
-  if (wg < 64u) wg = 64u;
-  if (wg > 1024u) wg = 1024u;
   if (dev_.maxWorkGroupSize && wg > dev_.maxWorkGroupSize)
-    wg = (uint32_t) (dev_.maxWorkGroupSize / 64u) * 64u;
+    wg = (uint32_t) std::min<uint64_t>(dev_.maxWorkGroupSize, 0xffffffffu);
+  wg = std::max<uint32_t>(64u, std::min<uint32_t>(wg, 1024u));
+  while (wg & (wg - 1u))
+    wg &= wg - 1u;
   wgSize_ = wg;
 
-  segmentsPerLaunch_ = config_.segmentsPerLaunch ? config_.segmentsPerLaunch : 65536u;
+  // segments per persistent launch (one slab); the reference's default is
+  // 256 per compute unit (src/gpu/cuda/CudaBackend.cpp:315-319)
+  segmentsPerLaunch_ = config_.segmentsPerLaunch
+                     ? config_.segmentsPerLaunch
+                     : std::max<uint32_t>(dev_.computeUnits, 1u) * 256u;
   bucketFactor_ = (config_.bucketFactor == 0) ? 7u
                 : (config_.bucketFactor == 0xffffffffu) ? 0u : config_.bucketFactor;
   // a slab spans up to 4 GiB of sieve bytes (one launch of each kernel
@@ -363,8 +372,13 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
             ? primesUpTo(std::max<uint64_t>((uint64_t) S * bucketFactor_, 1u << 16))
             : (uint32_t) hostPrimes_.size();
   const uint32_t windowSegs = std::max<uint32_t>(1, std::min(opt.windowSegs, PSG_MAX_WINDOW_SEGS));
+  const double hBig = invPrefix_[bigBegin_];
 
   // ---- runs -----------------------------------------------------------------
+  // A run never expects more bucket entries than a slab may hold: slabs are
+  // cut between runs, so a long interval with a heavy bucket load (high
+  // starts: ~4.9e5 entries per segment at 1e19) becomes several runs, and
+  // several slabs, instead of one slab past the 2^32-entry index range.
   runs_.clear();
   uint64_t segsTotal = 0;
   for (const Interval& v : iv)
@@ -375,7 +389,11 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
     const ByteRange br = byteRange(lo, v.hi);
     const uint32_t np = primesUpTo(isqrt64(v.hi));
     const uint64_t nseg = (br.gHi - br.gLo) / S + 1;
-    for (uint64_t s = 0; s < nseg; s += opt.runSegs)
+    const double perSeg = (np > bigBegin_) ? 8.0 * S * (invPrefix_[np] - hBig) : 0.0;
+    uint32_t runSegs = std::max<uint32_t>(1, opt.runSegs);
+    if (perSeg > 0 && perSeg * runSegs > opt.entryBudget)
+      runSegs = (uint32_t) std::max(1.0, std::floor(opt.entryBudget / perSeg));
+    for (uint64_t s = 0; s < nseg; s += runSegs)
     {
       PsgRun r;
       std::memset(&r, 0, sizeof r);
@@ -386,7 +404,7 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
       r.hiMask = br.hiMask;
       r.nprimes = np;
       r.out = v.out;
-      r.nseg = (uint32_t) std::min<uint64_t>(opt.runSegs, nseg - s);
+      r.nseg = (uint32_t) std::min<uint64_t>(runSegs, nseg - s);
       runs_.push_back(r);
     }
     segsTotal += nseg;
@@ -399,7 +417,6 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
   wins_.clear();
   segExpect_.clear();
   {
-    const double hBig = invPrefix_[bigBegin_];
     Slab cur{ 0, 0, 0, 0, 0, 0, 0, 0.0 };
     for (size_t i = 0; i < runs_.size(); i++)
     {
@@ -650,9 +667,12 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
 
 void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
 {
+  // a slab (one persistent launch of each kernel) holds at most
+  // segmentsPerLaunch_ segments, the reference's bound on a launch
+  // (src/gpu/cuda/CudaBackend.cpp:466)
   ExecOptions opt;
-  opt.runSegs = slabSegments_;
-  opt.slabSegs = slabSegments_;
+  opt.runSegs = std::min(slabSegments_, segmentsPerLaunch_);
+  opt.slabSegs = opt.runSegs;
   opt.windowSegs = windowSegs_;
   execute({ Interval{ lo, hi, 0 } }, 1, out, opt);
 }
@@ -660,11 +680,11 @@ void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
 void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6)
 {
   ExecOptions opt;
-  opt.runSegs = std::min<uint32_t>(2048, std::max<uint32_t>(slabSegments_, 1));
+  opt.runSegs = std::min<uint32_t>({ 2048u, std::max<uint32_t>(slabSegments_, 1), segmentsPerLaunch_ });
   opt.windowSegs = PSG_MAX_WINDOW_SEGS;  // few prime sweeps per run
   opt.primesPerChunk = 4096;
   opt.maxChunks = 16384;
-  opt.slabSegs = std::max<uint32_t>(opt.runSegs, batchSlabSegs_);
+  opt.slabSegs = std::min(std::max<uint32_t>(opt.runSegs, batchSlabSegs_), segmentsPerLaunch_);
   opt.pipeline = true;
   execute(iv, outs, out6, opt);
 }

@@ -12,7 +12,9 @@ Two kinds of data end up here:
 The config-3 query list (config3_queries.npy) is drawn from a fixed seed and
 frozen as data so that expected counts never depend on floating-point pow().
 
-Usage: python tests/golden/make_golden.py   (needs oracle/_ref built: make -C oracle ref)
+Usage: python tests/golden/make_golden.py [extra]   (needs oracle/_ref built: make -C oracle ref;
+       "extra" writes only cases_high.json, config3_decades.json and
+       config5_expected.json)
 """
 import ctypes
 import json
@@ -121,10 +123,70 @@ def config3_queries(n=8192, seed=20260, path=None):
     return q
 
 
+def high_cases(fns):
+    """Six-count fixtures above 2^63, where the reference GPU backend leaves
+    its division-magic path (src/gpu/cuda/CudaBackend.cpp:409-411): 1e9-wide
+    windows at 1e19, across 2^63 and ending at 2^64 - 1, seeded windows in
+    [2^63, 2^64), and short ranges whose last byte holds values past 2^64."""
+    top = 2 ** 64 - 1
+    spans = [(10 ** 19, 10 ** 19 + 10 ** 9), (2 ** 63 - 5 * 10 ** 8, 2 ** 63 + 5 * 10 ** 8),
+             (top - 10 ** 9, top), (top - 1000, top), (top - 31, top), (top - 30, top - 1),
+             (top, top), (top - 59, top - 29)]
+    rng = np.random.Generator(np.random.PCG64(63))
+    for _ in range(8):
+        a = 2 ** 63 + int(rng.integers(0, 2 ** 63 - 2 * 10 ** 8))
+        spans.append((a, a + int(rng.integers(0, 10 ** 8))))
+    return [{"start": a, "stop": b, "counts": {str(t): fns[t](a, b) for t in range(6)}}
+            for a, b in spans]
+
+
+def config3_decade_subset(fns, q, per_decade=16, skip=()):
+    """Reference counts for the first `per_decade` config-3 queries of every
+    start decade 10^9 .. 10^17 (lengths up to 2^32), beyond those in `skip`."""
+    out, seen = {}, {}
+    for i in range(len(q)):
+        if str(i) in skip:
+            continue
+        d = len(str(int(q[i, 0]))) - 1
+        if seen.get(d, 0) >= per_decade:
+            continue
+        seen[d] = seen.get(d, 0) + 1
+        out[str(i)] = fns[int(q[i, 2]) - 1](int(q[i, 0]), int(q[i, 1]))
+    return out
+
+
+def config5_subsets(fns):
+    """Reference counts of Poisson serving streams (primesieve-gpu_amd/serve.py
+    poisson_workload): all 400 queries of the test stream (seed 11) and every
+    40th query of the benchmark's 8000-query stream (seed 5)."""
+    sys.path.insert(0, os.path.join(ROOT, "primesieve-gpu_amd"))
+    import serve
+    out = {}
+    for seed, n, step in [(11, 400, 1), (5, 8000, 40)]:
+        _, qs = serve.poisson_workload(n, 2000.0, seed=seed)
+        out["%d:%d" % (seed, n)] = {str(i): fns[k - 1](a, b)
+                                    for i, (a, b, k) in enumerate(qs) if i % step == 0}
+    return out
+
+
 def main():
     if not os.path.exists(REF_LIB):
         sys.exit("build the reference library first: make -C oracle ref")
     fns = ref_lib()
+    if len(sys.argv) > 1 and sys.argv[1] == "extra":
+        # the fixtures added later, written without rewriting the others
+        q = np.load(os.path.join(HERE, "config3_queries.npy"), allow_pickle=False)
+        with open(os.path.join(HERE, "cases_high.json"), "w") as f:
+            json.dump(high_cases(fns), f)
+        print("wrote cases_high.json", flush=True)
+        skip = json.load(open(os.path.join(HERE, "config3_expected.json")))
+        with open(os.path.join(HERE, "config3_decades.json"), "w") as f:
+            json.dump(config3_decade_subset(fns, q, skip=skip), f)
+        print("wrote config3_decades.json", flush=True)
+        with open(os.path.join(HERE, "config5_expected.json"), "w") as f:
+            json.dump(config5_subsets(fns), f)
+        print("wrote config5_expected.json", flush=True)
+        return
 
     # the transcribed KATs must agree with the reference library itself
     for e in KATS:

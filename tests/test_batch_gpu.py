@@ -31,6 +31,21 @@ def test_config3_subset_matches_reference(gpu, mode):
     assert not bad, bad[:5]
 
 
+def test_config3_every_decade_matches_reference(gpu):
+    """16 fixture queries from every start decade 1e9 .. 1e17 (lengths up to
+    2^32, so up to 1e18 + 2^32) with reference-computed counts, batched and
+    plain."""
+    q = np.load(os.path.join(GOLDEN, "config3_queries.npy"), allow_pickle=False)
+    exp = json.load(open(os.path.join(GOLDEN, "config3_decades.json")))
+    idx = sorted(int(i) for i in exp)
+    qs = [(int(q[i, 0]), int(q[i, 1]), int(q[i, 2])) for i in idx]
+    print("queries", len(qs), "max start", max(a for a, _, _ in qs), flush=True)
+    for mode in (0, 1):
+        got = gpu.count_batch(qs, mode=mode)
+        bad = [(i, g, exp[str(i)]) for i, g in zip(idx, got) if g != exp[str(i)]]
+        assert not bad, (mode, bad[:5])
+
+
 def test_small_and_edge_queries_against_oracle(gpu, oracle):
     rng = random.Random(23)
     qs = [(10, 9, 1), (0, 0, 1), (0, 1, 2), (2, 2, 1), (3, 5, 2), (5, 17, 5), (0, 30, 6),

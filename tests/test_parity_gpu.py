@@ -50,12 +50,12 @@ def test_reference_case_list_single_pass(gpu, golden):
 
 
 def test_known_pi_and_kats(gpu, golden):
-    """Known values from the reference test suite (kats.json); ranges the
-    per-segment tiers finish quickly."""
+    """Known values from the reference test suite (kats.json), including the
+    five 1e11-wide chunks from 1e19 (src/app/stressTest.cpp:95-118): above
+    2^63, where the sieving primes reach 3.2e9 and each call runs as several
+    bucket slabs."""
     bad = []
     for e in golden("kats.json"):
-        if e["start"] >= 10 ** 18:
-            continue   # needs the big-prime bucket tier (tested separately)
         _say("kat", e["start"], e["stop"], e["type"])
         got = gpu.count(e["start"], e["stop"], e["type"])
         if got != e["count"]:
@@ -90,7 +90,59 @@ def test_random_ranges_golden(gpu, golden):
     assert not bad, bad[:5]
 
 
-@pytest.mark.parametrize("kb,wg", [(16, 256), (32, 512), (64, 1024), (128, 512), (150, 256), (160, 1024)])
+def test_high_ranges_golden(gpu, golden):
+    """[2^63, 2^64): six counts of windows at 1e19, across 2^63, up to
+    2^64 - 1 (whose last byte holds values past 2^64) and seeded windows,
+    against the reference CPU library."""
+    bad = []
+    for c in golden("cases_high.json"):
+        _say("high", c["start"], c["stop"])
+        got = gpu.count_all(c["start"], c["stop"])
+        if got != [c["counts"][str(t)] for t in range(6)]:
+            bad.append((c["start"], c["stop"], got, c["counts"]))
+    assert not bad, bad[:5]
+    top = golden("cases_high.json")[3]          # [2^64 - 1001, 2^64 - 1]
+    for t in range(6):
+        assert gpu.count(top["start"], top["stop"], t) == top["counts"][str(t)]
+
+
+def test_work_group_size_power_of_two(gpu, oracle):
+    """Workgroup sizes are rounded down to a power of two, as the reference
+    requires (src/gpu/GpuBackend.hpp:49-50, src/gpu/cuda/CudaBackend.cpp:260-263);
+    the counts of 192- and 768-thread requests equal the oracle's."""
+    try:
+        for req, want in [(192, 128), (768, 512), (960, 512), (100, 64)]:
+            gpu.set_work_group_size(req)
+            a, b = 10 ** 12 + 17, 10 ** 12 + 3 * 10 ** 7
+            got = gpu.count_all(a, b)
+            assert gpu.last_stats().work_group_size == want, req
+            assert got == oracle.count6(a, b), (req, got)
+    finally:
+        gpu.set_work_group_size(0)
+
+
+def test_segments_per_launch_bounds_launches(gpu, golden):
+    """set_gpu_segments_per_launch bounds the segments of one launch (the
+    reference's meaning, src/gpu/cuda/CudaBackend.cpp:315-319, 466): a
+    1e10-wide interval of 2544 segments of 128 KiB takes ceil(2544 / n)
+    launches, with unchanged counts."""
+    want = [e["count"] for e in golden("kats.json")
+            if e["start"] == 10 ** 12 and e["stop"] == 10 ** 12 + 10 ** 10 and e["type"] == 0][0]
+    try:
+        for n in [1000, 300]:
+            gpu.set_segments_per_launch(n)
+            assert gpu.count(10 ** 12, 10 ** 12 + 10 ** 10, 0) == want
+            st = gpu.last_stats()
+            _say("segments per launch", n, st.as_dict())
+            assert st.segments == 2544 and st.launches == -(-2544 // n)
+    finally:
+        gpu.set_segments_per_launch(0)
+    gpu.count(10 ** 12, 10 ** 12 + 10 ** 10, 0)
+    assert gpu.last_stats().launches == 1
+
+
+@pytest.mark.parametrize("kb,wg", [(16, 256), (32, 512), (64, 1024), (128, 512), (150, 256), (160, 1024),
+                                   (64, 192), (128, 768)])
 def test_geometries_against_oracle(gpu, oracle, kb, wg):
     """Every segment size / workgroup size gives the oracle's counts,
     including ranges that straddle segment ends and byte ends."""
