@@ -174,6 +174,17 @@ def workload_batch(m, model):
         raise SystemExit("cold and warm batched counts differ")
     res["batched_stats"] = {k: v for k, v in m.last_stats().as_dict().items()
                             if k in ("kernel_ms", "fill_ms", "device_ms", "launches", "segments")}
+    # the same batch through a captured launch series: the first call
+    # captures it, the second replays it
+    t0 = time.perf_counter()
+    g1 = m.count_batch(qs, mode=m.BATCH_GRAPH)
+    res["graph_capture_batched_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g2 = m.count_batch(qs, mode=m.BATCH_GRAPH)
+    res["graph_batched_s"] = time.perf_counter() - t0
+    if g1 != batched or g2 != batched or m.last_stats().graph != 1:
+        raise SystemExit("graph-replayed batch differs from the eager batch")
+    res["graph_batched_intervals_per_s"] = len(qs) / res["graph_batched_s"]
     t0 = time.perf_counter()
     plain = m.count_batch(qs, mode=m.BATCH_PLAIN)
     res["plain_s"] = time.perf_counter() - t0
@@ -241,8 +252,9 @@ def workload_serving(m, model, seconds=4.0, qps=2000.0):
     arrivals, qs = serve.poisson_workload(n, qps, seed=5)
     m.count_all(10 ** 18, 10 ** 18 + 10 ** 6)
     counts, lat, wall, batches = serve.serve(arrivals, qs, model=model)
+    gcounts, glat, gwall, gbatches = serve.serve(arrivals, qs, model=model, mode=m.BATCH_GRAPH)
     check = m.count_batch(qs, mode=m.BATCH_PLAIN)
-    if check != counts:
+    if check != counts or gcounts != counts:
         raise SystemExit("served counts differ from the plain path")
     exp = json.load(open(os.path.join(ROOT, "tests", "golden", "config5_expected.json")))
     exp = exp.get("5:%d" % n, {})
@@ -253,6 +265,8 @@ def workload_serving(m, model, seconds=4.0, qps=2000.0):
     s["workload"] = "Poisson %.0f QPS for %.0f s, Pareto(1.2) lengths in [2^12, 2^36], k 1..6" % (qps, seconds)
     s["counts_checked"] = ("all %d against the plain per-query path, %d against the reference CPU library"
                            % (n, len(exp)))
+    g = serve.summary(glat, gwall, n, gbatches)
+    s["graph_replay"] = {k: g[k] for k in ("throughput_qps", "p50_ms", "p99_ms", "max_ms", "batches")}
     return s
 
 

@@ -97,7 +97,8 @@ int psg_gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6]);
  * [start_i, stop_i], k = 1 primes, 2 twins, ..., 6 sextuplets.
  * PSG_BATCH_PERSISTENT packs all queries into persistent work-stealing
  * launches; PSG_BATCH_PLAIN runs one count call (launch series + sync) per
- * query, the baseline the batch is compared with.  0 on success, -1 on error. */
+ * query, the baseline the batch is compared with; PSG_BATCH_GRAPH is below.
+ * 0 on success, -1 on error. */
 typedef struct psg_query {
   uint64_t start;
   uint64_t stop;
@@ -106,6 +107,12 @@ typedef struct psg_query {
 } psg_query;
 #define PSG_BATCH_PERSISTENT 0
 #define PSG_BATCH_PLAIN 1
+/* persistent, through a captured launch series (hipGraph): the series of a
+ * batch (clear, then per slab bucket fill and segment kernel) takes all its
+ * arguments from one device-resident descriptor upload, so it is captured
+ * once per shape (slab count, which slabs fill buckets) and replayed for
+ * every later batch of that shape with new descriptors. */
+#define PSG_BATCH_GRAPH 2
 int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int mode);
 
 /* ---- device-resident sieving primes -------------------------------------
@@ -136,7 +143,9 @@ typedef struct psg_stats {
   uint32_t work_group_size;  /* threads per workgroup */
   uint32_t sieving_primes;   /* sieving primes struck in LDS (from 53; from 7 without pre-sieve) with p*p <= stop */
   uint32_t bucket_retries;   /* calls repeated because a bucket list outgrew its room */
-  uint32_t reserved;
+  uint32_t graph;            /* 0 launched one by one (kernel_ms / fill_ms timed per launch);
+                                1 replayed a captured launch series, 2 captured it first
+                                (then only device_ms is timed) */
 } psg_stats;
 int psg_gpu_last_stats(psg_stats* out);
 

@@ -29,7 +29,7 @@ HIP_FLAGS = ["--genco", "--offload-arch=gfx950", "-O3", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter",
              "-D__HIP_PLATFORM_AMD__=1", "-I" + os.path.join(ROOT, "include"),
-             "-I" + ROCM_INC]
+             "-I" + os.path.join(CSRC, "boundary"), "-I" + ROCM_INC]
 
 KERNELS = {"psg_sieve": "PSG_SIEVE_HSACO", "psg_bench": "PSG_BENCH_HSACO",
            "psg_mlp": "PSG_MLP_HSACO"}

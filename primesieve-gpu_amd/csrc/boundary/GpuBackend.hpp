@@ -1,13 +1,22 @@
 ///
 /// @file  GpuBackend.hpp
-/// @brief The backend boundary, kept as the reference defines it
-///        (src/gpu/GpuBackend.hpp:26-98): DeviceInfo, GpuConfig, the abstract
+/// @brief The backend boundary of the count path, kept member for member as
+///        the reference defines it (src/gpu/GpuBackend.hpp:26-98): DeviceInfo,
+///        GpuConfig (its four tunables and nothing else), the abstract
 ///        GpuBackend with count(lo >= 7, hi, countType), listDevices() and
-///        createBackend().  There is exactly one implementation, the gfx950
-///        HipBackend; no per-vendor #if dispatch.
+///        createBackend().  The three class definitions are layout- and
+///        member-identical to the reference's, so the HIP backend compiles
+///        against either header and one program may hold both.  The
+///        gfx950 HipBackend is the only implementation; no per-vendor #if
+///        dispatch.  Its own extra knobs (bucket tier, slab size) are not
+///        part of this boundary: see psg::HipTuning in HipBackend.hpp.
 ///
-#ifndef PSG_GPU_BACKEND_HPP
-#define PSG_GPU_BACKEND_HPP
+///        This header sits in its own include directory (not beside
+///        HipBackend.hpp) so that a build can put the reference's
+///        src/gpu/GpuBackend.hpp in its place.
+///
+#ifndef PRIMESIEVE_GPU_BACKEND_HPP
+#define PRIMESIEVE_GPU_BACKEND_HPP
 
 #include <stdint.h>
 #include <memory>
@@ -39,15 +48,14 @@ struct DeviceInfo
 /// Tunables; 0 means "derive a default from the device".
 struct GpuConfig
 {
-  uint32_t sieveBytes = 0;         // LDS bytes per segment (multiple of 256)
-  uint32_t workGroupSize = 0;      // threads per workgroup, multiple of 64
-  uint32_t segmentsPerLaunch = 0;  // segments per kernel launch
-  bool preSieve = true;            // pre-sieve tiles for 7..47
-  uint32_t slabSegments = 0;       // segments per big-prime bucket slab
-  uint32_t bucketFactor = 0;       // primes > bucketFactor * sieveBytes use the
-                                   // bucket tier; 0 = auto (7), ~0u = never
-  uint32_t bucketMargin = 0;       // bucket list room beyond the expected count,
-                                   // in standard deviations; 0 = auto (6)
+  /// LDS bytes per segment (rounded down to a multiple of 256).
+  uint32_t sieveBytes = 0;
+  /// Threads per workgroup, rounded down to a power of two (64 .. 1024).
+  uint32_t workGroupSize = 0;
+  /// Segments per persistent launch (one slab); 0 = 256 per compute unit.
+  uint32_t segmentsPerLaunch = 0;
+  /// Pre-sieve tiles for the primes 7..47.
+  bool preSieve = true;
 };
 
 class GpuBackend

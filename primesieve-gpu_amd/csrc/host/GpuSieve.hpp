@@ -11,7 +11,7 @@
 
 #include <primesieve/gpu.hpp>
 
-namespace psg { class HipBackend; }
+namespace psg { class HipBackend; struct HipTuning; }
 
 namespace primesieve {
 namespace gpu {
@@ -19,8 +19,9 @@ namespace gpu {
 class GpuBackend;
 struct GpuConfig;
 
-/// Replace the whole configuration; the backend is recreated on next use.
-void setConfig(const GpuConfig& config);
+/// Replace the whole configuration (the reference's four tunables and the
+/// HIP backend's own); the backend is recreated on next use.
+void setTuning(const GpuConfig& config, const psg::HipTuning& tuning);
 
 /// The global lock every public GPU entry point holds.
 std::mutex& gpuMutex();
@@ -35,7 +36,8 @@ std::string& lastErrorRef();
 void resetBackend();
 /// Six counts of [start, stop] including 2/3/5; false when no GPU is usable.
 bool countAllOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6]);
-/// Batch counts (mode 0 persistent, 1 plain); false when no GPU is usable.
+/// Batch counts (mode 0 persistent, 1 plain, 2 persistent through a captured
+/// launch series); false when no GPU is usable.
 bool countBatchOnGpu(const GpuQuery* q, size_t n, uint64_t* out, int mode);
 
 } // namespace gpu

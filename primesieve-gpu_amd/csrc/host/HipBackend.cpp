@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 // gfx950 code object of kernels/psg_sieve.hip, linked in by embed.S
@@ -51,8 +52,28 @@ void DeviceBuffer::release()
   bytes_ = 0;
 }
 
-HipBackend::HipBackend(const HipApi* api, const DeviceInfo& dev, const GpuConfig& config)
-  : api_(api), dev_(dev), config_(config)
+namespace {
+
+std::mutex g_tuningMutex;
+HipTuning g_tuning;
+
+} // namespace
+
+void setHipTuning(const HipTuning& t)
+{
+  std::lock_guard<std::mutex> lock(g_tuningMutex);
+  g_tuning = t;
+}
+
+HipTuning hipTuning()
+{
+  std::lock_guard<std::mutex> lock(g_tuningMutex);
+  return g_tuning;
+}
+
+HipBackend::HipBackend(const HipApi* api, const DeviceInfo& dev, const GpuConfig& config,
+                       const HipTuning& tuning)
+  : api_(api), dev_(dev), config_(config), hipTuning_(tuning)
 {
   init();
 }
@@ -62,6 +83,8 @@ HipBackend::~HipBackend()
   if (!api_)
     return;
   (void) api_->SetDevice(dev_.deviceIndex);
+  for (const CachedGraph& g : graphs_)
+    (void) api_->GraphExecDestroy(g.exec);
   primes_.release();
   magics_.release();
   tiles_.release();
@@ -114,16 +137,16 @@ void HipBackend::init()
   segmentsPerLaunch_ = config_.segmentsPerLaunch
                      ? config_.segmentsPerLaunch
                      : std::max<uint32_t>(dev_.computeUnits, 1u) * 256u;
-  bucketFactor_ = (config_.bucketFactor == 0) ? 7u
-                : (config_.bucketFactor == 0xffffffffu) ? 0u : config_.bucketFactor;
+  bucketFactor_ = (hipTuning_.bucketFactor == 0) ? 7u
+                : (hipTuning_.bucketFactor == 0xffffffffu) ? 0u : hipTuning_.bucketFactor;
   // a slab spans up to 4 GiB of sieve bytes (one launch of each kernel
   // for a 1e11 interval: no launch tails between slabs; its bucket entries,
   // 4 B per crossing of a big prime and at most the entry budget, fit in HBM)
-  uint32_t slab = config_.slabSegments ? config_.slabSegments
-                                       : (uint32_t) ((1ull << 32) / sieveBytes_);
+  uint32_t slab = hipTuning_.slabSegments ? hipTuning_.slabSegments
+                                          : (uint32_t) ((1ull << 32) / sieveBytes_);
   slabSegments_ = std::max<uint32_t>(1, std::min<uint32_t>(slab, 32768));
   windowSegs_ = 128;
-  bucketMargin_ = config_.bucketMargin ? config_.bucketMargin : 6u;
+  bucketMargin_ = hipTuning_.bucketMargin ? hipTuning_.bucketMargin : 6u;
   firstCrossPrime_ = config_.preSieve ? PSG_FIRST_CROSS_PRIME : 7u;
 
   const size_t coBytes = (size_t) (psg_sieve_hsaco_end - psg_sieve_hsaco_start);
@@ -131,10 +154,23 @@ void HipBackend::init()
            ("loading the gfx950 code object (" + std::to_string(coBytes) + " bytes)").c_str());
   const struct { hipFunction_t* f; const char* name; } fns[] = {
     { &fnSieve_, "psg_sieve_count" }, { &fnBucketFill_, "psg_bucket_fill" },
-    { &fnMagics_, "psg_prime_magics" } };
+    { &fnMagics_, "psg_prime_magics" }, { &fnClear_, "psg_clear" } };
   for (const auto& f : fns)
     hipCheck(api_, api_->ModuleGetFunction(f.f, module_, f.name),
              (std::string("hipModuleGetFunction(") + f.name + ")").c_str());
+  // The code object's static scratch array (the device memory of the
+  // no-allocation entry point): its address on this device, resolved once
+  // here with the device current.  Host code uses only this address.
+  {
+    hipDeviceptr_t p = nullptr;
+    size_t bytes = 0;
+    hipCheck(api_, api_->ModuleGetGlobal(&p, &bytes, module_, "psg_scratch"),
+             "hipModuleGetGlobal(psg_scratch)");
+    if (!p || bytes < PSG_SCRATCH_BYTES)
+      throw primesieve_error("psg_scratch: unexpected size " + std::to_string(bytes));
+    scratchAddr_ = (uint64_t) (uintptr_t) p;
+    scratchBytes_ = PSG_SCRATCH_BYTES;
+  }
   // the segment kernel addresses its sieve words by LDS offset, which holds
   // only while its dynamic LDS starts at 0, i.e. it has no static LDS
   int staticLds = -1;
@@ -336,6 +372,21 @@ uint32_t HipBackend::persistentGrid(uint32_t nSegs)
   return std::max<uint32_t>(1, std::min(nSegs, persistentWGs_));
 }
 
+/// Workgroups of a persistent bucket-fill launch for nItems work items: as
+/// many as the device holds at once, fewer when there is less work.
+uint32_t HipBackend::fillGrid(uint32_t nItems)
+{
+  if (!fillWGs_)
+  {
+    int perCU = 0;
+    const uint32_t lds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
+    if (api_->ModuleOccupancy(&perCU, fnBucketFill_, (int) PSG_BUCKET_THREADS, lds) != hipSuccess || perCU < 1)
+      perCU = 1;
+    fillWGs_ = (uint32_t) perCU * (dev_.computeUnits ? dev_.computeUnits : 256);
+  }
+  return std::max<uint32_t>(1, std::min(nItems, fillWGs_));
+}
+
 /// Sieve every interval; out6[slot * 6 + t] receives the six counts of the
 /// intervals whose `out` is slot.  Intervals are cut into runs of at most
 /// opt.runSegs segments, runs are packed into slabs, and each slab is one
@@ -360,7 +411,16 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
     if (v.lo <= v.hi && v.hi >= 7) maxHi = std::max(maxHi, v.hi);
   if (!maxHi)
     return;
-  ensureSievingPrimes((uint32_t) isqrt64(maxHi));
+  if (opt.noAlloc)
+  {
+    // the sieving-prime cache is device memory: it must be there already
+    if (hostPrimes_.empty() || isqrt64(maxHi) > primesLimit_)
+      throw primesieve_error("the sieving primes up to " + std::to_string(isqrt64(maxHi)) +
+                             " are not cached on the device; a call that may allocate "
+                             "(psg_gpu_count_all or psg_gpu_sieving_primes) must cache them first");
+  }
+  else
+    ensureSievingPrimes((uint32_t) isqrt64(maxHi));
 
   const uint32_t S = sieveBytes_;
   uint32_t shift = 0;
@@ -469,7 +529,8 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
   if (!launchPlan(outs, room, opt))
   {
     // a list outgrew its room: the fill counted every list exactly
-    hipCheck(api_, api_->Memcpy(room.data(), listCount_.get(), room.size() * 4, hipMemcpyDeviceToHost),
+    hipCheck(api_, api_->Memcpy(room.data(), (const void*) (uintptr_t) listCountAddr_, room.size() * 4,
+                                hipMemcpyDeviceToHost),
              "hipMemcpy bucket list counts");
     stats_.bucket_retries = 1;
     if (!launchPlan(outs, room, opt))
@@ -485,7 +546,25 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
 
 /// Launch the current plan with the given bucket list rooms (one per
 /// segment of the call).  False when a list outgrew its room (the counts are
-/// then incomplete and listCount_ holds the exact list sizes).
+/// then incomplete and the list counters at listCountAddr_ hold the exact
+/// list sizes).
+///
+/// Every argument the kernels take is in one host-built blob, uploaded with a
+/// single copy (the PSG_DESC_* layout of kernel_abi.h): the clear block, one
+/// record per slab, then the run, window and list-offset tables.  A kernel's
+/// only parameter is the address of its record, so a launch series depends on
+/// the plan only through its slab count and which slabs fill buckets.  Three
+/// forms:
+///   * eager: psg_clear, then per slab psg_bucket_fill (when it has big
+///     primes) and psg_sieve_count, each timed with events; a batch with two
+///     or more bucket slabs fills slab k+1 on a second stream, into the other
+///     entry buffer, while slab k is sieved;
+///   * graph (opt.graph): the same series on one stream with fixed persistent
+///     grids, captured once per shape and replayed; no memset nodes (the
+///     clear kernel zeroes counters and counts);
+///   * no allocation (opt.noAlloc): every per-call buffer is carved out of the
+///     code object's psg_scratch array (resolved at init); one stream, one
+///     entry buffer reused by every slab.
 bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, const ExecOptions& opt)
 {
   const uint32_t S = sieveBytes_;
@@ -512,40 +591,83 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
     maxEntries = std::max(maxEntries, acc);
   }
 
-  // ---- device buffers and one upload of every descriptor ------------------------
-  const size_t runBytes = runs_.size() * sizeof(PsgRun);
-  const size_t winBytes = wins_.size() * sizeof(PsgWindow);
-  const size_t offBytes = listOff.size() * 4;
+  // ---- sizes -----------------------------------------------------------------
+  const size_t nSlabs = slabs_.size();
   const size_t nSegsAll = room.size();
-  descriptors_.allocate(api_, runBytes + winBytes + offBytes + 16);
-  staging_.resize(runBytes + winBytes + offBytes);
-  std::memcpy(staging_.data(), runs_.data(), runBytes);
-  if (winBytes) std::memcpy(staging_.data() + runBytes, wins_.data(), winBytes);
-  std::memcpy(staging_.data() + runBytes + winBytes, listOff.data(), offBytes);
-  hipCheck(api_, api_->MemcpyAsync(descriptors_.get(), staging_.data(), staging_.size(),
-                                   hipMemcpyHostToDevice, stream_), "hipMemcpyAsync descriptors");
-  const uint64_t offBase = descriptors_.addr() + runBytes + winBytes;
-  counts_.allocate(api_, (size_t) outs * PSG_NCOUNTS * sizeof(uint64_t));
-  workCounters_.allocate(api_, slabs_.size() * 4);
-  listCount_.allocate(api_, nSegsAll * 4 + 4);
-  // Pipelined (batches): with two or more bucket slabs, the fill of slab k+1
-  // runs on its own stream while slab k is sieved, into the other entry
-  // buffer.  Neither kernel's workgroup fits in LDS beside the other's, so
-  // the fill takes the CUs that the segment kernel's workgroups leave at the
-  // end of a slab: the overlap is the launch tails.
+  const size_t runOff = PSG_DESC_HEADER + nSlabs * PSG_SLAB_RECORD;
+  const size_t winOff = runOff + runs_.size() * sizeof(PsgRun);
+  const size_t offOff = winOff + wins_.size() * sizeof(PsgWindow);
+  const size_t descBytes = offOff + listOff.size() * 4;
+  const size_t countBytes = (size_t) outs * PSG_NCOUNTS * sizeof(uint64_t);
+  const size_t workBytes = nSlabs * 8;             // per slab: sieve and fill work counters
+  const size_t listCountBytes = nSegsAll * 4 + 4;
+  const size_t entryBytes = (size_t) maxEntries * 4;
   size_t bucketSlabs = 0;
   for (const Slab& sl : slabs_)
     bucketSlabs += (sl.win1 > sl.win0) ? 1 : 0;
-  const bool pipelined = opt.pipeline && bucketSlabs >= 2;
-  if (maxEntries)
-    entries_[0].allocate(api_, (size_t) maxEntries * 4);
-  if (maxEntries && pipelined)
-    entries_[1].allocate(api_, (size_t) maxEntries * 4);
-  flags_.allocate(api_, 16);
-  hipCheck(api_, api_->MemsetAsync(counts_.get(), 0, (size_t) outs * PSG_NCOUNTS * 8, stream_), "hipMemsetAsync");
-  hipCheck(api_, api_->MemsetAsync(workCounters_.get(), 0, slabs_.size() * 4, stream_), "hipMemsetAsync");
-  hipCheck(api_, api_->MemsetAsync(listCount_.get(), 0, nSegsAll * 4, stream_), "hipMemsetAsync");
-  hipCheck(api_, api_->MemsetAsync(flags_.get(), 0, 16, stream_), "hipMemsetAsync");
+  // Pipelined (eager batches): neither kernel's workgroup fits in LDS beside
+  // the other's, so the fill of slab k+1 takes the CUs that the segment
+  // kernel's workgroups leave at the end of slab k: the overlap is the
+  // launch tails.
+  const bool pipelined = opt.pipeline && !opt.graph && !opt.noAlloc && bucketSlabs >= 2;
+
+  // ---- device memory -----------------------------------------------------------
+  uint64_t desc, counts, work, listCount, flags, entries[2] = { 0, 0 };
+  if (opt.noAlloc)
+  {
+    // the scratch array: flags | counts | work counters | list counters |
+    // descriptors | bucket entries, each 256-byte aligned
+    uint64_t at = 0;
+    const auto carve = [&](size_t bytes) {
+      const uint64_t a = scratchAddr_ + at;
+      at += (bytes + 255) & ~(uint64_t) 255;
+      return a;
+    };
+    flags = carve(16);
+    counts = carve(countBytes);
+    work = carve(workBytes);
+    listCount = carve(listCountBytes);
+    desc = carve(descBytes);
+    entries[0] = carve(entryBytes);
+    if (at > scratchBytes_)
+      throw primesieve_error("count needs " + std::to_string(at) + " bytes of device scratch; the "
+                             "no-allocation entry point has " + std::to_string(scratchBytes_) +
+                             " (use psg_gpu_count_all for this interval)");
+  }
+  else
+  {
+    descriptors_.allocate(api_, descBytes);
+    counts_.allocate(api_, countBytes);
+    workCounters_.allocate(api_, workBytes);
+    listCount_.allocate(api_, listCountBytes);
+    flags_.allocate(api_, 16);
+    if (maxEntries)
+      entries_[0].allocate(api_, entryBytes);
+    if (maxEntries && pipelined)
+      entries_[1].allocate(api_, entryBytes);
+    desc = descriptors_.addr();
+    counts = counts_.addr();
+    work = workCounters_.addr();
+    listCount = listCount_.addr();
+    flags = flags_.addr();
+    entries[0] = entries_[0].addr();
+    entries[1] = entries_[1].addr();
+  }
+  listCountAddr_ = listCount;
+
+  // ---- the descriptor blob ----------------------------------------------------
+  staging_.assign(descBytes, 0);
+  unsigned char* blob = staging_.data();
+  PsgClearArgs clr;
+  std::memset(&clr, 0, sizeof clr);
+  clr.addr[0] = counts;    clr.words[0] = (uint32_t) (countBytes / 4);
+  clr.addr[1] = work;      clr.words[1] = (uint32_t) (workBytes / 4);
+  clr.addr[2] = listCount; clr.words[2] = (uint32_t) nSegsAll;
+  clr.addr[3] = flags;     clr.words[3] = 4;
+  static_assert(sizeof(PsgClearArgs) <= PSG_DESC_HEADER, "clear block");
+  static_assert(sizeof(PsgSieveArgs) <= PSG_SLAB_FILL_OFFSET, "sieve record");
+  static_assert(PSG_SLAB_FILL_OFFSET + sizeof(PsgBucketArgs) <= PSG_SLAB_RECORD, "fill record");
+  std::memcpy(blob, &clr, sizeof clr);
 
   PsgSieveArgs args;
   std::memset(&args, 0, sizeof args);
@@ -553,7 +675,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
   args.magics = magics_.addr();
   args.tiles = tiles_.addr();
   args.tilesInLds = (tiles_.addr() && tilesFitLds()) ? 1u : 0u;
-  args.counts = counts_.addr();
+  args.counts = counts;
   args.segBytes = S;
   args.tier0End = primesUpTo(std::min<uint32_t>(PSG_TIER0_MAX_PRIME, S / 8));
   args.tier1End = primesUpTo(std::min<uint32_t>(PSG_TIER1_MAX_PRIME, S));
@@ -565,82 +687,151 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
   PsgBucketArgs b;
   std::memset(&b, 0, sizeof b);
   b.primes = primes_.addr();
-  b.overflow = flags_.addr();
-  b.total = flags_.addr() + 8;
+  b.overflow = flags;
+  b.total = flags + 8;
   b.segShift = shift;
   b.bigBegin = bigBegin_;
-  const uint32_t fillLds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
 
-  hipCheck(api_, api_->EventRecord(evStart_, stream_), "hipEventRecord");
-  hipStream_t fs = stream_;
-  if (pipelined)
-  {
-    // the fills start after the copies and clears above
-    fs = fillStream_;
-    hipCheck(api_, api_->EventRecord(evSetup_, stream_), "hipEventRecord");
-    hipCheck(api_, api_->StreamWaitEvent(fs, evSetup_, 0), "hipStreamWaitEvent");
-  }
-  // events: fill begin/end on fs, sieve begin/end on stream_, per slab
-  struct SlabEvents { hipEvent_t fb, fe, sb, se; };
-  std::vector<SlabEvents> ev(slabs_.size(), SlabEvents{ nullptr, nullptr, nullptr, nullptr });
-  size_t nev = 0;
-  hipEvent_t lastUse[2] = { nullptr, nullptr };   // last sieve reading each entry buffer
+  std::vector<uint32_t> parityOf(nSlabs, 0);
   uint32_t parity = 0;
-  for (size_t si = 0; si < slabs_.size(); si++)
+  for (size_t si = 0; si < nSlabs; si++)
   {
     const Slab& sl = slabs_[si];
-    const uint64_t slabOff = offBase + offPos[si] * 4;
-    const uint64_t slabCount = listCount_.addr() + (uint64_t) sl.segBase * 4;
     const bool buckets = sl.win1 > sl.win0;
-    if (buckets)
-    {
-      if (pipelined && lastUse[parity])
-        hipCheck(api_, api_->StreamWaitEvent(fs, lastUse[parity], 0), "hipStreamWaitEvent");
-      b.entries = entries_[parity].addr();
-      b.windows = descriptors_.addr() + runBytes + sl.win0 * sizeof(PsgWindow);
-      b.nWindows = (uint32_t) (sl.win1 - sl.win0);
-      b.nWGs = sl.nWGs;
-      b.listOff = slabOff;
-      b.listCount = slabCount;
-      void* pb[] = { &b };
-      ev[si].fb = phaseEvent(nev++);
-      ev[si].fe = phaseEvent(nev++);
-      hipCheck(api_, api_->EventRecord(ev[si].fb, fs), "hipEventRecord");
-      hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, sl.nWGs, 1, 1, PSG_BUCKET_THREADS, 1, 1,
-                                              fillLds, fs, pb, nullptr),
-               "hipModuleLaunchKernel(psg_bucket_fill)");
-      hipCheck(api_, api_->EventRecord(ev[si].fe, fs), "hipEventRecord");
-      if (pipelined)
-        hipCheck(api_, api_->StreamWaitEvent(stream_, ev[si].fe, 0), "hipStreamWaitEvent");
-      args.buckets = entries_[parity].addr();
-    }
-    else
-      args.buckets = 0;
-    args.listOff = slabOff;
-    args.listCount = slabCount;
-    args.runs = descriptors_.addr() + sl.run0 * sizeof(PsgRun);
+    const uint64_t slabOff = desc + offOff + offPos[si] * 4;
+    const uint64_t slabCount = listCount + (uint64_t) sl.segBase * 4;
+    parityOf[si] = parity;
+    args.runs = desc + runOff + sl.run0 * sizeof(PsgRun);
     args.nRuns = (uint32_t) (sl.run1 - sl.run0);
     args.nSegs = sl.nSegs;
-    args.workCounter = workCounters_.addr() + si * 4;
-    void* pa[] = { &args };
-    ev[si].sb = phaseEvent(nev++);
-    ev[si].se = phaseEvent(nev++);
-    hipCheck(api_, api_->EventRecord(ev[si].sb, stream_), "hipEventRecord");
-    hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, persistentGrid(sl.nSegs), 1, 1, wgSize_, 1, 1,
-                                            sieveLds(), stream_, pa, nullptr),
+    args.buckets = buckets ? entries[parity] : 0;
+    args.listOff = slabOff;
+    args.listCount = slabCount;
+    args.workCounter = work + si * 8;
+    b.windows = desc + winOff + sl.win0 * sizeof(PsgWindow);
+    b.nWindows = (uint32_t) (sl.win1 - sl.win0);
+    b.nItems = buckets ? sl.nWGs : 0;
+    b.listOff = slabOff;
+    b.listCount = slabCount;
+    b.entries = entries[parity];
+    b.workCounter = work + si * 8 + 4;
+    unsigned char* rec = blob + PSG_DESC_HEADER + si * PSG_SLAB_RECORD;
+    std::memcpy(rec, &args, sizeof args);
+    std::memcpy(rec + PSG_SLAB_FILL_OFFSET, &b, sizeof b);
+    if (buckets && pipelined)
+      parity ^= 1u;
+  }
+  std::memcpy(blob + runOff, runs_.data(), runs_.size() * sizeof(PsgRun));
+  if (!wins_.empty())
+    std::memcpy(blob + winOff, wins_.data(), wins_.size() * sizeof(PsgWindow));
+  std::memcpy(blob + offOff, listOff.data(), listOff.size() * 4);
+  hipCheck(api_, api_->MemcpyAsync((void*) (uintptr_t) desc, blob, descBytes, hipMemcpyHostToDevice, stream_),
+           "hipMemcpyAsync descriptors");
+
+  // ---- launches -----------------------------------------------------------------
+  const uint32_t fillLds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
+  const bool fixedGrids = opt.graph;
+  const auto recordAddr = [&](size_t si) { return desc + PSG_DESC_HEADER + si * PSG_SLAB_RECORD; };
+  const auto launchClear = [&](hipStream_t st) {
+    uint64_t a = desc;
+    void* pa[] = { &a };
+    hipCheck(api_, api_->ModuleLaunchKernel(fnClear_, kClearWGs, 1, 1, 256, 1, 1, 0, st, pa, nullptr),
+             "hipModuleLaunchKernel(psg_clear)");
+  };
+  const auto launchFill = [&](size_t si, hipStream_t st) {
+    uint64_t a = recordAddr(si) + PSG_SLAB_FILL_OFFSET;
+    void* pa[] = { &a };
+    const uint32_t grid = fixedGrids ? fillGrid(~0u) : fillGrid(slabs_[si].nWGs);
+    hipCheck(api_, api_->ModuleLaunchKernel(fnBucketFill_, grid, 1, 1, PSG_BUCKET_THREADS, 1, 1,
+                                            fillLds, st, pa, nullptr),
+             "hipModuleLaunchKernel(psg_bucket_fill)");
+  };
+  const auto launchSieve = [&](size_t si, hipStream_t st) {
+    uint64_t a = recordAddr(si);
+    void* pa[] = { &a };
+    const uint32_t grid = fixedGrids ? persistentGrid(~0u) : persistentGrid(slabs_[si].nSegs);
+    hipCheck(api_, api_->ModuleLaunchKernel(fnSieve_, grid, 1, 1, wgSize_, 1, 1, sieveLds(), st,
+                                            pa, nullptr),
              "hipModuleLaunchKernel(psg_sieve_count)");
-    hipCheck(api_, api_->EventRecord(ev[si].se, stream_), "hipEventRecord");
-    if (buckets)
+  };
+
+  struct SlabEvents { hipEvent_t fb, fe, sb, se; };
+  std::vector<SlabEvents> ev;
+  stats_.graph = 0;
+  hipCheck(api_, api_->EventRecord(evStart_, stream_), "hipEventRecord");
+  if (opt.graph)
+  {
+    std::vector<uint64_t> key = { desc, nSlabs, persistentGrid(~0u), fillGrid(~0u), sieveLds() };
+    for (const Slab& sl : slabs_)
+      key.push_back(sl.win1 > sl.win0);
+    bool captured = false;
+    hipGraphExec_t exec = graphFor(key, captured, [&]() {
+      launchClear(stream_);
+      for (size_t si = 0; si < nSlabs; si++)
+      {
+        if (slabs_[si].win1 > slabs_[si].win0)
+          launchFill(si, stream_);
+        launchSieve(si, stream_);
+      }
+    });
+    hipCheck(api_, api_->GraphLaunch(exec, stream_), "hipGraphLaunch");
+    stats_.graph = captured ? 2u : 1u;
+  }
+  else if (opt.noAlloc)
+  {
+    launchClear(stream_);
+    for (size_t si = 0; si < nSlabs; si++)
     {
-      lastUse[parity] = ev[si].se;
-      if (pipelined) parity ^= 1u;
+      if (slabs_[si].win1 > slabs_[si].win0)
+        launchFill(si, stream_);
+      launchSieve(si, stream_);
+    }
+  }
+  else
+  {
+    launchClear(stream_);
+    hipStream_t fs = stream_;
+    if (pipelined)
+    {
+      // the fills start after the upload and the clear
+      fs = fillStream_;
+      hipCheck(api_, api_->EventRecord(evSetup_, stream_), "hipEventRecord");
+      hipCheck(api_, api_->StreamWaitEvent(fs, evSetup_, 0), "hipStreamWaitEvent");
+    }
+    // events: fill begin/end on fs, sieve begin/end on stream_, per slab
+    ev.assign(nSlabs, SlabEvents{ nullptr, nullptr, nullptr, nullptr });
+    size_t nev = 0;
+    hipEvent_t lastUse[2] = { nullptr, nullptr };   // the last sieve reading each entry buffer
+    for (size_t si = 0; si < nSlabs; si++)
+    {
+      const bool buckets = slabs_[si].win1 > slabs_[si].win0;
+      const uint32_t par = parityOf[si];
+      if (buckets)
+      {
+        if (pipelined && lastUse[par])
+          hipCheck(api_, api_->StreamWaitEvent(fs, lastUse[par], 0), "hipStreamWaitEvent");
+        ev[si].fb = phaseEvent(nev++);
+        ev[si].fe = phaseEvent(nev++);
+        hipCheck(api_, api_->EventRecord(ev[si].fb, fs), "hipEventRecord");
+        launchFill(si, fs);
+        hipCheck(api_, api_->EventRecord(ev[si].fe, fs), "hipEventRecord");
+        if (pipelined)
+          hipCheck(api_, api_->StreamWaitEvent(stream_, ev[si].fe, 0), "hipStreamWaitEvent");
+      }
+      ev[si].sb = phaseEvent(nev++);
+      ev[si].se = phaseEvent(nev++);
+      hipCheck(api_, api_->EventRecord(ev[si].sb, stream_), "hipEventRecord");
+      launchSieve(si, stream_);
+      hipCheck(api_, api_->EventRecord(ev[si].se, stream_), "hipEventRecord");
+      if (buckets)
+        lastUse[par] = ev[si].se;
     }
   }
   hipCheck(api_, api_->EventRecord(evStop_, stream_), "hipEventRecord");
   hostCounts_.resize((size_t) outs * PSG_NCOUNTS);
-  hipCheck(api_, api_->MemcpyAsync(hostFlags_, flags_.get(), 16, hipMemcpyDeviceToHost, stream_),
+  hipCheck(api_, api_->MemcpyAsync(hostFlags_, (void*) (uintptr_t) flags, 16, hipMemcpyDeviceToHost, stream_),
            "hipMemcpyAsync flags");
-  hipCheck(api_, api_->MemcpyAsync(hostCounts_.data(), counts_.get(), hostCounts_.size() * 8,
+  hipCheck(api_, api_->MemcpyAsync(hostCounts_.data(), (void*) (uintptr_t) counts, hostCounts_.size() * 8,
                                    hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync counts");
   hipCheck(api_, api_->StreamSynchronize(stream_), "sieve kernels");
 
@@ -665,6 +856,45 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
   return (hostFlags_[0] & 0xffffffffull) == 0;
 }
 
+/// The instantiated launch series for `key`, captured from `record` on the
+/// backend's stream the first time the key is seen (captured = true).  At
+/// most kMaxGraphs series are kept; the oldest is dropped first.
+hipGraphExec_t HipBackend::graphFor(const std::vector<uint64_t>& key, bool& captured,
+                                    const std::function<void()>& record)
+{
+  captured = false;
+  for (const CachedGraph& g : graphs_)
+    if (g.key == key)
+      return g.exec;
+  hipCheck(api_, api_->StreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+           "hipStreamBeginCapture");
+  try
+  {
+    record();
+  }
+  catch (...)
+  {
+    hipGraph_t g = nullptr;
+    if (api_->StreamEndCapture(stream_, &g) == hipSuccess && g)
+      (void) api_->GraphDestroy(g);
+    throw;
+  }
+  hipGraph_t graph = nullptr;
+  hipCheck(api_, api_->StreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = api_->GraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void) api_->GraphDestroy(graph);
+  hipCheck(api_, e, "hipGraphInstantiate");
+  if (graphs_.size() >= kMaxGraphs)
+  {
+    (void) api_->GraphExecDestroy(graphs_.front().exec);
+    graphs_.erase(graphs_.begin());
+  }
+  graphs_.push_back(CachedGraph{ key, exec });
+  captured = true;
+  return exec;
+}
+
 void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
 {
   // a slab (one persistent launch of each kernel) holds at most
@@ -677,9 +907,10 @@ void HipBackend::countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
   execute({ Interval{ lo, hi, 0 } }, 1, out, opt);
 }
 
-void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6)
+void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6, bool graph)
 {
   ExecOptions opt;
+  opt.graph = graph;
   opt.runSegs = std::min<uint32_t>({ 2048u, std::max<uint32_t>(slabSegments_, 1), segmentsPerLaunch_ });
   opt.windowSegs = PSG_MAX_WINDOW_SEGS;  // few prime sweeps per run
   opt.primesPerChunk = 4096;
@@ -760,7 +991,7 @@ std::unique_ptr<GpuBackend> createHipBackend(const DeviceInfo& dev,
   }
   try
   {
-    return std::unique_ptr<GpuBackend>(new psg::HipBackend(api, dev, config));
+    return std::unique_ptr<GpuBackend>(new psg::HipBackend(api, dev, config, psg::hipTuning()));
   }
   catch (const std::exception& e)
   {

@@ -22,6 +22,7 @@
 #include <primesieve_gpu.h>
 
 #include <stdint.h>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -63,6 +64,23 @@ struct Interval
   uint32_t out;
 };
 
+/// Knobs of the HIP backend beyond the reference's GpuConfig (which keeps
+/// exactly the reference's four members); 0 means auto.  A backend reads
+/// them when it is created (createHipBackend), so a change applies to the
+/// next backend, like a GpuConfig change.
+struct HipTuning
+{
+  uint32_t slabSegments = 0;   // segments per big-prime bucket slab (auto: 4 GiB of sieve bytes)
+  uint32_t bucketFactor = 0;   // primes > bucketFactor * sieve bytes use the bucket tier;
+                               // auto = 7, 0xffffffff = never
+  uint32_t bucketMargin = 0;   // bucket list room beyond the expected count, in
+                               // standard deviations; auto = 6
+};
+
+/// The process-wide HipTuning that new backends take (thread-safe).
+void setHipTuning(const HipTuning& t);
+HipTuning hipTuning();
+
 /// How the engine cuts work (all in segments).
 struct ExecOptions
 {
@@ -73,6 +91,8 @@ struct ExecOptions
   uint32_t maxChunks = 256;       // ... but at most maxChunks workgroups per window
   double entryBudget = 3.0e9;     // expected bucket entries per slab (12 GB)
   bool pipeline = false;          // fill slab k+1 on a second stream while slab k is sieved
+  bool graph = false;             // replay a captured launch series (captured once per shape)
+  bool noAlloc = false;           // per-call device memory from psg_scratch only
   uint64_t listOut = 0;           // list mode: uint32_t* for the primes (one segment per interval)
   uint64_t listOffsets = 0;       // list mode: const uint32_t[outs], first output index per interval
 };
@@ -81,7 +101,7 @@ class HipBackend : public primesieve::gpu::GpuBackend
 {
 public:
   HipBackend(const HipApi* api, const primesieve::gpu::DeviceInfo& dev,
-             const primesieve::gpu::GpuConfig& config);
+             const primesieve::gpu::GpuConfig& config, const HipTuning& tuning);
   ~HipBackend() override;
 
   const primesieve::gpu::DeviceInfo& device() const override { return dev_; }
@@ -93,8 +113,9 @@ public:
   void countAll(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS]);
 
   /// Six counts per output slot for a batch of intervals (lo >= 7 after
-  /// clamping), all in one launch series.
-  void countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6);
+  /// clamping), all in one launch series; with `graph` the series is a
+  /// captured one, replayed for every batch of the same shape.
+  void countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6, bool graph = false);
 
   const psg_stats& lastStats() const { return stats_; }
   /// Make the device cache hold every sieving prime up to `limit` and copy
@@ -127,6 +148,9 @@ private:
   void generatePrimesOnDevice(uint32_t want);
   uint32_t primesUpTo(uint64_t x) const;
   uint32_t persistentGrid(uint32_t nSegs);
+  uint32_t fillGrid(uint32_t nItems);
+  hipGraphExec_t graphFor(const std::vector<uint64_t>& key, bool& captured,
+                          const std::function<void()>& record);
   // LDS of the segment kernel: the segment, its tail and (when they fit) the
   // small pre-sieve tiles
   bool tilesFitLds() const
@@ -143,6 +167,7 @@ private:
   const HipApi* api_;
   primesieve::gpu::DeviceInfo dev_;
   primesieve::gpu::GpuConfig config_;
+  HipTuning hipTuning_;
   uint32_t sieveBytes_ = 0;
   uint32_t ldsBytes_ = 0;         // LDS per workgroup the device offers (capped)
   uint32_t wgSize_ = 0;
@@ -152,12 +177,28 @@ private:
   uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
   uint32_t windowSegs_ = 128;     // segments per bucket window of a single interval
   uint32_t persistentWGs_ = 0;    // resident workgroups of the sieve kernel
+  uint32_t fillWGs_ = 0;          // resident workgroups of the bucket fill
   uint32_t bucketMargin_ = 6;     // bucket list room: expected + margin * sqrt(expected) + 64
 
   hipModule_t module_ = nullptr;
   hipFunction_t fnSieve_ = nullptr;
   hipFunction_t fnBucketFill_ = nullptr;
   hipFunction_t fnMagics_ = nullptr;
+  hipFunction_t fnClear_ = nullptr;
+  static constexpr uint32_t kClearWGs = 256;   // grid of psg_clear (grid-stride)
+  uint64_t scratchAddr_ = 0;      // device address of psg_scratch on this device
+  uint64_t scratchBytes_ = 0;
+  uint64_t listCountAddr_ = 0;    // bucket list counters of the current call
+
+  // captured launch series, keyed by shape (descriptor address, slab count,
+  // grids, which slabs fill buckets)
+  struct CachedGraph
+  {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+  };
+  static constexpr size_t kMaxGraphs = 16;
+  std::vector<CachedGraph> graphs_;
   hipStream_t stream_ = nullptr;
   hipStream_t fillStream_ = nullptr;   // bucket fills of a pipelined call
   hipEvent_t evSetup_ = nullptr;

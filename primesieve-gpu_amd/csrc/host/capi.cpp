@@ -84,15 +84,16 @@ int psg_set_gpu_tuning(const psg_tuning* t)
 {
   if (!t)
     return -1;
-  primesieve::gpu::GpuConfig c;
+  primesieve::gpu::GpuConfig c;        // the reference's four tunables
   c.sieveBytes = t->sieve_bytes;
   c.workGroupSize = t->work_group_size;
   c.segmentsPerLaunch = t->segments_per_launch;
-  c.slabSegments = t->slab_segments;
-  c.bucketFactor = t->bucket_factor;
-  c.bucketMargin = t->bucket_margin;
   c.preSieve = t->presieve != 0;
-  primesieve::gpu::setConfig(c);
+  psg::HipTuning h;                     // the HIP backend's own
+  h.slabSegments = t->slab_segments;
+  h.bucketFactor = t->bucket_factor;
+  h.bucketMargin = t->bucket_margin;
+  primesieve::gpu::setTuning(c, h);
   return 0;
 }
 

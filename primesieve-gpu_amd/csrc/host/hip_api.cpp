@@ -68,6 +68,7 @@ void load()
   ok &= resolve(lib, "hipModuleLoadData", a.ModuleLoadData, missing);
   ok &= resolve(lib, "hipModuleUnload", a.ModuleUnload, missing);
   ok &= resolve(lib, "hipModuleGetFunction", a.ModuleGetFunction, missing);
+  ok &= resolve(lib, "hipModuleGetGlobal", a.ModuleGetGlobal, missing);
   ok &= resolve(lib, "hipModuleLaunchKernel", a.ModuleLaunchKernel, missing);
   ok &= resolve(lib, "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor", a.ModuleOccupancy, missing);
   ok &= resolve(lib, "hipFuncGetAttribute", a.FuncGetAttribute, missing);

@@ -36,6 +36,7 @@ struct HipApi
   decltype(&hipModuleLoadData) ModuleLoadData = nullptr;
   decltype(&hipModuleUnload) ModuleUnload = nullptr;
   decltype(&hipModuleGetFunction) ModuleGetFunction = nullptr;
+  decltype(&hipModuleGetGlobal) ModuleGetGlobal = nullptr;
   decltype(&hipModuleLaunchKernel) ModuleLaunchKernel = nullptr;
   decltype(&hipModuleOccupancyMaxActiveBlocksPerMultiprocessor) ModuleOccupancy = nullptr;
   decltype(&hipFuncGetAttribute) FuncGetAttribute = nullptr;

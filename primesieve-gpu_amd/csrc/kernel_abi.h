@@ -58,7 +58,11 @@ typedef struct PsgRun {
 } PsgRun;
 
 /* Arguments of psg_sieve_count (persistent: workgroups take slab segments
- * from *workCounter until nSegs is reached). */
+ * from *workCounter until nSegs is reached).  Like every argument block
+ * here it lives in device memory, uploaded with the call's run and window
+ * tables; the kernel's only parameter is its address.  A launch series thus
+ * depends on the plan only through the number of slabs, and one captured
+ * series replays for every plan of that shape. */
 typedef struct PsgSieveArgs {
   uint64_t runs;        /* const PsgRun*, sorted by segStart */
   uint64_t primes;      /* const uint32_t*: sieving primes, ascending */
@@ -109,7 +113,10 @@ typedef struct PsgWindow {
   uint32_t pad;
 } PsgWindow;
 
-/* Arguments of psg_bucket_fill (one workgroup per (window, chunk)). */
+/* Arguments of psg_bucket_fill: the work items are the (window, chunk)
+ * pairs, numbered 0 .. nItems-1 (window w owns items wgBase .. wgBase +
+ * nChunks - 1); the workgroups of a persistent grid take them from
+ * *workCounter. */
 typedef struct PsgBucketArgs {
   uint64_t windows;     /* const PsgWindow*, sorted by wgBase */
   uint64_t primes;      /* const uint32_t*; every bucket prime is >= 2^16 */
@@ -118,10 +125,31 @@ typedef struct PsgBucketArgs {
   uint64_t entries;     /* uint32_t* */
   uint64_t overflow;    /* uint32_t flag, set when a list outgrows its room */
   uint64_t total;       /* unsigned long long: entries appended (statistics) */
+  uint64_t workCounter; /* uint32_t*, zero at launch */
   uint32_t nWindows;
-  uint32_t nWGs;        /* workgroups of the fill grid */
+  uint32_t nItems;      /* (window, chunk) work items; 0 = nothing to fill */
   uint32_t segShift;    /* log2(segBytes) */
   uint32_t bigBegin;
 } PsgBucketArgs;
+
+/* Device memory that psg_clear zeroes before a slab series: up to four
+ * ranges of 32-bit words (counts, work counters, list counts, flags). */
+#define PSG_CLEAR_RANGES 4
+typedef struct PsgClearArgs {
+  uint64_t addr[PSG_CLEAR_RANGES];
+  uint32_t words[PSG_CLEAR_RANGES];
+} PsgClearArgs;
+
+/* Layout of a call's descriptor upload: the clear block, then one 256-byte
+ * record per slab (its PsgSieveArgs, then its PsgBucketArgs), then the run,
+ * window and list-offset tables the records point into. */
+#define PSG_DESC_HEADER 64u
+#define PSG_SLAB_RECORD 256u
+#define PSG_SLAB_FILL_OFFSET 128u
+
+/* The static scratch array of the code object (psg_scratch): the device
+ * memory of count calls that must not allocate (descriptors, counters,
+ * bucket lists). */
+#define PSG_SCRATCH_BYTES (64u << 20)
 
 #endif

@@ -60,6 +60,22 @@ __device__ __forceinline__ const PSG_GLOBAL T* gptr(uint64_t a) { return (const 
 template <typename T>
 __device__ __forceinline__ PSG_GLOBAL T* gptr_rw(uint64_t a) { return (PSG_GLOBAL T*) (uintptr_t) a; }
 
+// An argument block of kernel_abi.h read from device memory at a uniform
+// address: constant-address-space loads, so its fields land in scalar
+// registers once per workgroup.
+#define PSG_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ T load_args(uint64_t addr)
+{
+  static_assert(sizeof(T) % 4 == 0, "argument blocks are whole 32-bit words");
+  union { T v; uint32_t w[sizeof(T) / 4]; } u;
+  const PSG_CONST uint32_t* src = (const PSG_CONST uint32_t*) (uintptr_t) addr;
+#pragma unroll
+  for (uint32_t k = 0; k < sizeof(T) / 4; k++)
+    u.w[k] = src[k];
+  return u.v;
+}
+
 __device__ __constant__ uint32_t kOffsets[8] = {7, 11, 13, 17, 19, 23, 29, 31};
 
 // R = B mod p for B < 2^63 using M = ceil(2^(64+L)/p):
@@ -1022,21 +1038,48 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
   return n;
 }
 
-__device__ __forceinline__ const PsgWindow& window_of_wg(const PsgBucketArgs& a, uint32_t& chunk)
+// The window of fill work item `item` and the item's chunk in it.
+__device__ __forceinline__ PsgWindow window_of_item(const PsgBucketArgs& a, uint32_t item, uint32_t& chunk)
 {
   const auto* win = gptr<PsgWindow>(a.windows);
-  const uint32_t w = last_le(a.nWindows, blockIdx.x, [&](uint32_t i) { return win[i].wgBase; });
-  chunk = blockIdx.x - win[w].wgBase;
+  const uint32_t w = last_le(a.nWindows, item, [&](uint32_t i) { return win[i].wgBase; });
+  chunk = item - win[w].wgBase;
   return win[w];
 }
 
 }  // namespace
 
-// Persistent segment kernel: each workgroup takes slab segments from
-// *workCounter (work stealing) until all nSegs are done.
-extern "C" __global__ void __launch_bounds__(1024)
-psg_sieve_count(PsgSieveArgs args)
+// The scratch memory of count calls that must not allocate: the host
+// resolves its address once per loaded code object (hipModuleGetGlobal) and
+// carves the call's descriptors, counters and bucket lists out of it.  It
+// has external linkage so that the module loader can find it by name.
+extern "C" {
+__device__ __attribute__((used)) __align__(256) uint32_t psg_scratch[PSG_SCRATCH_BYTES / 4];
+}
+
+// Zero the ranges of a PsgClearArgs block (counts, work counters, bucket
+// list counters, flags) before a slab series -- a kernel rather than
+// memset calls, so that a captured launch series holds kernel nodes only.
+extern "C" __global__ void __launch_bounds__(256)
+psg_clear(uint64_t argsAddr)
 {
+  const PsgClearArgs c = load_args<PsgClearArgs>(argsAddr);
+  const uint32_t stride = gridDim.x * blockDim.x;
+#pragma unroll
+  for (uint32_t r = 0; r < PSG_CLEAR_RANGES; r++) {
+    auto* p = gptr_rw<uint32_t>(c.addr[r]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c.words[r]; i += stride)
+      p[i] = 0u;
+  }
+}
+
+// Persistent segment kernel: each workgroup takes slab segments from
+// *workCounter (work stealing) until all nSegs are done.  Its argument
+// block is in device memory (PsgSieveArgs at argsAddr).
+extern "C" __global__ void __launch_bounds__(1024)
+psg_sieve_count(uint64_t argsAddr)
+{
+  const PsgSieveArgs args = load_args<PsgSieveArgs>(argsAddr);
   // the sieve starts at LDS address 0 (strike addresses need no base); the
   // class table and the block's small variables follow it
   extern __shared__ uint32_t lds[];
@@ -1142,16 +1185,20 @@ psg_sieve_count(PsgSieveArgs args)
 
 constexpr uint32_t kStageF = PSG_BUCKET_HITS;   // hits per thread per stage
 
-// One pass per (window, chunk) workgroup, in stages.  In a stage every thread
-// walks up to kStageF hits into its LDS stage slots while an LDS histogram
-// counts them per window segment.  Then each segment's run of the stage takes
-// its place in the segment's list with one global atomic (issued early, used
-// late), the hits are sorted by segment in place (every thread first holds
-// its hits in registers), and the sorted runs are written out contiguously:
-// coalesced stores instead of one scattered store per hit.
-extern "C" __global__ void __launch_bounds__(PSG_BUCKET_THREADS)
-psg_bucket_fill(PsgBucketArgs a)
+// Persistent: the workgroups take (window, chunk) work items from
+// *workCounter until all nItems are done.  One pass per item, in stages.
+// In a stage every thread walks up to kStageF hits into its LDS stage slots
+// while an LDS histogram counts them per window segment.  Then each
+// segment's run of the stage takes its place in the segment's list with one
+// global atomic (issued early, used late), the hits are sorted by segment in
+// place (every thread first holds its hits in registers), and the sorted
+// runs are written out contiguously: coalesced stores instead of one
+// scattered store per hit.  Its argument block is in device memory
+// (PsgBucketArgs at argsAddr).
+extern "C" __global__ void __launch_bounds__(PSG_BUCKET_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+psg_bucket_fill(uint64_t argsAddr)
 {
+  const PsgBucketArgs a = load_args<PsgBucketArgs>(argsAddr);
   extern __shared__ uint32_t lds[];
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = PSG_BUCKET_THREADS;        // the launch's block size
@@ -1161,104 +1208,113 @@ psg_bucket_fill(PsgBucketArgs a)
   uint64_t* place = (uint64_t*) (hist + PSG_MAX_WINDOW_SEGS);
   __shared__ __align__(16) uint32_t ctab[240];
   __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
+  __shared__ uint32_t sItem;
 
   init_ctab(ctab);
-  uint32_t c;
-  const PsgWindow w = window_of_wg(a, c);
-  const uint32_t W = w.segs;                           // <= PSG_MAX_WINDOW_SEGS <= blockDim
-  const auto* __restrict__ listOff = gptr<uint32_t>(a.listOff) + w.firstSeg;
-  uint32_t* listCount = (uint32_t*) a.listCount + w.firstSeg;
   auto* entries = gptr_rw<uint32_t>(a.entries);
   const uint32_t lane = tid & 63u, wave = tid >> 6, nwaves = nthreads >> 6;
-  for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
-
   const uint32_t eshift = a.segShift + 3u;             // window bit index -> window segment
   const uint32_t segBits = (1u << eshift) - 1u;        // window bit index -> segment bit index
-  BucketWalk st;
-  st.k = a.bigBegin + c * nthreads + tid;
-  st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
-  st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
-  const uint32_t stride = nthreads * w.nChunks;
   uint64_t appended = 0;
-  __syncthreads();
   for (;;) {
-    const uint32_t n = bucket_walk_stage(a, w, st, stride, ctab, kStageF,
-        [&](uint32_t i, uint32_t seg, uint32_t hit) {
-          stage[i * nthreads + tid] = hit;
-          atomicAdd(&hist[seg], 1u);
-        });
-    const int more = __syncthreads_or((st.p != 0u || st.pn != 0u) ? 1 : 0);
-
-    // exclusive prefix of the counts; thread i owns segment i
-    const uint32_t cnt = (tid < W) ? hist[tid] : 0u;
-    uint32_t x = cnt;
-    for (uint32_t d = 1; d < 64u; d <<= 1) {
-      const uint32_t y = (uint32_t) __shfl_up((int) x, d);
-      if (lane >= d) x += y;
-    }
-    if (lane == 63u) waveTotal[wave] = x;
+    // The next item.  Every thread has read the previous item's number before
+    // thread 0 overwrites it: the item's stages end with a barrier.
+    if (tid == 0) sItem = atomicAdd((uint32_t*) a.workCounter, 1u);
     __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (uint32_t v = 0; v < nwaves; v++) {
-      const uint32_t t = waveTotal[v];
-      before += (v < wave) ? t : 0u;
-      total += t;
-    }
-    const uint32_t f = before + x - cnt;
-    uint32_t at = 0;
-    if (tid < W) {
-      hist[tid] = f;
-      if (cnt) at = atomicAdd(&listCount[tid], cnt);   // consumed after the scatter
-    }
-    uint32_t e[kStageF];
-#pragma unroll
-    for (uint32_t u = 0; u < kStageF; u++) e[u] = stage[u * nthreads + tid];
-    __syncthreads();                                   // cursors set, stage read
-
-    // Scatter by segment, eight hits at a time: their cursor atomics first,
-    // then their stores, so that eight atomics are in flight at once.
-#pragma unroll
-    for (uint32_t g = 0; g < kStageF; g += 8) {
-      uint32_t slot[8];
-#pragma unroll
-      for (uint32_t q = 0; q < 8; q++)
-        slot[q] = (g + q < n) ? atomicAdd(&hist[e[g + q] >> eshift], 1u) : 0u;
-#pragma unroll
-      for (uint32_t q = 0; q < 8; q++)
-        if (g + q < n) stage[slot[q]] = e[g + q];
-    }
-    if (tid < W) {
-      const uint32_t room = listOff[tid + 1u] - listOff[tid];
-      if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
-      const uint32_t fit = (at >= room) ? 0u : min(cnt, room - at);
-      place[tid] = (uint64_t) (f | (fit << 16)) | ((uint64_t) (listOff[tid] + at) << 32);
-    }
-    __syncthreads();
-
-    // Write-out, eight slots at a time: their stage reads, then their
-    // segments' (place, destination) reads, then the stores.  Slots past
-    // `total` hold stale values; their segment index is masked into range
-    // and they store nothing.
-#pragma unroll
-    for (uint32_t g = 0; g < kStageF; g += 8) {
-      uint32_t v[8];
-      uint64_t pl[8];
-#pragma unroll
-      for (uint32_t q = 0; q < 8; q++) v[q] = stage[(g + q) * nthreads + tid];
-#pragma unroll
-      for (uint32_t q = 0; q < 8; q++) pl[q] = place[(v[q] >> eshift) & (PSG_MAX_WINDOW_SEGS - 1u)];
-#pragma unroll
-      for (uint32_t q = 0; q < 8; q++) {
-        const uint32_t i = (g + q) * nthreads + tid;
-        const uint32_t first = (uint32_t) pl[q], dst = (uint32_t) (pl[q] >> 32);
-        const uint32_t r = i - (first & 0xffffu);
-        if (i < total && r < (first >> 16)) entries[dst + r] = v[q] & segBits;
-      }
-    }
-    appended += total;
+    const uint32_t item = sItem;
+    if (item >= a.nItems) break;
+    uint32_t c;
+    const PsgWindow w = window_of_item(a, item, c);
+    const uint32_t W = w.segs;                           // <= PSG_MAX_WINDOW_SEGS <= blockDim
+    const auto* __restrict__ listOff = gptr<uint32_t>(a.listOff) + w.firstSeg;
+    uint32_t* listCount = (uint32_t*) a.listCount + w.firstSeg;
     for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
+
+    BucketWalk st;
+    st.k = a.bigBegin + c * nthreads + tid;
+    st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
+    st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
+    const uint32_t stride = nthreads * w.nChunks;
     __syncthreads();
-    if (!more) break;
+    for (;;) {
+      const uint32_t n = bucket_walk_stage(a, w, st, stride, ctab, kStageF,
+          [&](uint32_t i, uint32_t seg, uint32_t hit) {
+            stage[i * nthreads + tid] = hit;
+            atomicAdd(&hist[seg], 1u);
+          });
+      const int more = __syncthreads_or((st.p != 0u || st.pn != 0u) ? 1 : 0);
+
+      // exclusive prefix of the counts; thread i owns segment i
+      const uint32_t cnt = (tid < W) ? hist[tid] : 0u;
+      uint32_t x = cnt;
+      for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = (uint32_t) __shfl_up((int) x, d);
+        if (lane >= d) x += y;
+      }
+      if (lane == 63u) waveTotal[wave] = x;
+      __syncthreads();
+      uint32_t before = 0, total = 0;
+      for (uint32_t v = 0; v < nwaves; v++) {
+        const uint32_t t = waveTotal[v];
+        before += (v < wave) ? t : 0u;
+        total += t;
+      }
+      const uint32_t f = before + x - cnt;
+      uint32_t at = 0;
+      if (tid < W) {
+        hist[tid] = f;
+        if (cnt) at = atomicAdd(&listCount[tid], cnt);   // consumed after the scatter
+      }
+      uint32_t e[kStageF];
+#pragma unroll
+      for (uint32_t u = 0; u < kStageF; u++) e[u] = stage[u * nthreads + tid];
+      __syncthreads();                                   // cursors set, stage read
+
+      // Scatter by segment, eight hits at a time: their cursor atomics first,
+      // then their stores, so that eight atomics are in flight at once.
+#pragma unroll
+      for (uint32_t g = 0; g < kStageF; g += 8) {
+        uint32_t slot[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++)
+          slot[q] = (g + q < n) ? atomicAdd(&hist[e[g + q] >> eshift], 1u) : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++)
+          if (g + q < n) stage[slot[q]] = e[g + q];
+      }
+      if (tid < W) {
+        const uint32_t room = listOff[tid + 1u] - listOff[tid];
+        if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
+        const uint32_t fit = (at >= room) ? 0u : min(cnt, room - at);
+        place[tid] = (uint64_t) (f | (fit << 16)) | ((uint64_t) (listOff[tid] + at) << 32);
+      }
+      __syncthreads();
+
+      // Write-out, eight slots at a time: their stage reads, then their
+      // segments' (place, destination) reads, then the stores.  Slots past
+      // `total` hold stale values; their segment index is masked into range
+      // and they store nothing.
+#pragma unroll
+      for (uint32_t g = 0; g < kStageF; g += 8) {
+        uint32_t v[8];
+        uint64_t pl[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) v[q] = stage[(g + q) * nthreads + tid];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) pl[q] = place[(v[q] >> eshift) & (PSG_MAX_WINDOW_SEGS - 1u)];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) {
+          const uint32_t i = (g + q) * nthreads + tid;
+          const uint32_t first = (uint32_t) pl[q], dst = (uint32_t) (pl[q] >> 32);
+          const uint32_t r = i - (first & 0xffffu);
+          if (i < total && r < (first >> 16)) entries[dst + r] = v[q] & segBits;
+        }
+      }
+      appended += total;
+      for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
+      __syncthreads();
+      if (!more) break;
+    }
   }
   if (tid == 0 && appended)
     atomicAdd((unsigned long long*) a.total, (unsigned long long) appended);

@@ -38,7 +38,7 @@ class Stats(ctypes.Structure):
         ("work_group_size", ctypes.c_uint32),
         ("sieving_primes", ctypes.c_uint32),
         ("bucket_retries", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("graph", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -76,6 +76,7 @@ class Query(ctypes.Structure):
 
 BATCH_PERSISTENT = 0
 BATCH_PLAIN = 1
+BATCH_GRAPH = 2
 
 
 class Tuning(ctypes.Structure):

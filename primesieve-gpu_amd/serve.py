@@ -54,9 +54,12 @@ def bucket_entries_estimate(start, stop, seg_bytes=131072, factor=7):
     return 8.0 * nbytes * (math.log(math.log(hi)) - math.log(math.log(lo)))
 
 
-def serve(arrivals, queries, model=None, entry_budget=2.0e8, max_batch=4096, log=None):
-    """Run the loop in real time.  Returns (counts, latencies in seconds,
-    wall seconds, batches)."""
+def serve(arrivals, queries, model=None, entry_budget=2.0e8, max_batch=4096, log=None,
+          mode=psgpu.BATCH_PERSISTENT):
+    """Run the loop in real time.  mode: psgpu.BATCH_PERSISTENT launches each
+    batch's series one by one, psgpu.BATCH_GRAPH replays a captured series
+    (captured once per shape).  Returns (counts, latencies in seconds, wall
+    seconds, batches)."""
     n = len(queries)
     counts = [None] * n
     lat = np.zeros(n)
@@ -88,7 +91,7 @@ def serve(arrivals, queries, model=None, entry_budget=2.0e8, max_batch=4096, log
         else:
             order = np.argsort([-(b - a) for a, b, _ in qs], kind="stable")
         batch = [batch[j] for j in order]
-        out = psgpu.count_batch([queries[i] for i in batch])
+        out = psgpu.count_batch([queries[i] for i in batch], mode=mode)
         done = time.perf_counter() - t0
         for i, c in zip(batch, out):
             counts[i] = c

@@ -99,3 +99,38 @@ def test_batch_over_several_bucket_slabs(gpu):
     for i, s in enumerate(starts):
         want = gpu.count_all(s + 7 * i, s + 7 * i + span)
         assert got[6 * i:6 * i + 6] == want, (s, got[6 * i:6 * i + 6], want)
+
+
+def test_graph_replay_matches_reference_and_eager(gpu, oracle):
+    """PSG_BATCH_GRAPH: the batch's launch series (clear, bucket fill,
+    segment kernel per slab) is captured once per shape and replayed with
+    new descriptors.  Every replay -- with eager batches and count calls run
+    between them -- must equal the reference's counts and the eager
+    persistent path, for two different shapes."""
+    q = np.load(os.path.join(GOLDEN, "config3_queries.npy"), allow_pickle=False)
+    exp = json.load(open(os.path.join(GOLDEN, "config3_expected.json")))
+    exp.update(json.load(open(os.path.join(GOLDEN, "config3_decades.json"))))
+    idx = sorted(int(i) for i in exp)
+    qs = [(int(q[i, 0]), int(q[i, 1]), int(q[i, 2])) for i in idx]
+    want = [exp[str(i)] for i in idx]
+    eager = gpu.count_batch(qs)
+    assert eager == want
+    rng = random.Random(5)
+    small = []
+    for _ in range(40):
+        a = rng.randrange(10 ** 9, 10 ** 12)
+        small.append((a, a + rng.randrange(10 ** 6), rng.randrange(1, 7)))
+    small_want = [oracle.count6(a, b)[k - 1] for a, b, k in small]
+    seen = set()
+    for rep in range(3):
+        got = gpu.count_batch(qs, mode=gpu.BATCH_GRAPH)
+        st = gpu.last_stats()
+        print("replay", rep, st.as_dict(), flush=True)
+        seen.add(st.graph)
+        assert got == want, rep
+        # eager work between replays
+        assert gpu.count_batch(small) == small_want
+        assert gpu.count_all(10 ** 12, 10 ** 12 + 10 ** 8)[0] == 3618282
+        got_small = gpu.count_batch(small, mode=gpu.BATCH_GRAPH)
+        assert got_small == small_want, rep
+    assert 1 in seen      # at least one pure replay of a captured series

@@ -123,12 +123,12 @@ def test_launch_failure_mid_count(fake_hip):
         assert m.active_device() != ""
         m.set_enabled(True)
         assert m.try_count(0, 10**9) is None
-        assert "psg_sieve_count" in m.last_error(), m.last_error()
+        assert "hipModuleLaunchKernel(psg_" in m.last_error(), m.last_error()
         try:
             m.count(0, 10**9)
             raise SystemExit("count must fail")
         except m.PrimesieveError as e:
-            assert "psg_sieve_count" in str(e)
+            assert "hipModuleLaunchKernel(psg_" in str(e)
         assert m.lib().psg_gpu_count_twins(0, 10**9) == m.PSG_ERROR
         print("OK")
     """)

@@ -28,3 +28,17 @@ def test_serving_loop_counts_are_exact(gpu, oracle):
     s = serve.summary(lat, wall, len(qs), batches)
     print("serving summary", s, flush=True)
     assert s["p99_ms"] >= s["p50_ms"] > 0 and batches >= 1
+
+
+def test_serving_loop_graph_replay(gpu):
+    """The same stream served through captured launch series (each batch
+    replays the series captured for its shape) gives the reference's counts."""
+    import serve
+    arrivals, qs = serve.poisson_workload(400, qps=2000.0, seed=11)
+    counts, lat, wall, batches = serve.serve(arrivals, qs, mode=gpu.BATCH_GRAPH)
+    exp = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                      "config5_expected.json")))["11:400"]
+    bad = [(i, qs[i], counts[i], v) for i, v in ((int(i), v) for i, v in exp.items()) if counts[i] != v]
+    assert not bad, bad[:5]
+    s = serve.summary(lat, wall, len(qs), batches)
+    print("graph serving summary", s, flush=True)
