@@ -115,6 +115,22 @@ typedef struct psg_query {
 #define PSG_BATCH_GRAPH 2
 int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int mode);
 
+/* ---- no device allocation ------------------------------------------------
+ * No reference counterpart.  For callers that must not allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): psg_gpu_count_all with every per-call device buffer
+ * (descriptors, counters, bucket lists, counts) carved out of one static
+ * array of the backend's code object, psg_scratch, PSG_SCRATCH_BYTES
+ * (64 MiB) per device, held while the backend lives.  The call makes no
+ * hipMalloc / hipFree: the backend and its sieving-prime cache must already
+ * exist (any earlier count call, or psg_gpu_sieving_primes(isqrt(stop)),
+ * creates them); otherwise, or when the interval needs more scratch than
+ * there is (about 1.6e13 integers at most, fewer at high starts), it returns
+ * -1 with the reason in psg_gpu_last_error.  Calls are serialised by the
+ * library's lock; overlapping calls on one GPU from two processes each use
+ * their own copy of the array.  0 on success. */
+int psg_gpu_count_all_noalloc(uint64_t start, uint64_t stop, uint64_t counts[6]);
+
 /* ---- device-resident sieving primes -------------------------------------
  * No reference counterpart: the reference regenerates its sieving primes on
  * the CPU whenever sqrt(stop) grows (src/gpu/cuda/CudaBackend.cpp:346-388).

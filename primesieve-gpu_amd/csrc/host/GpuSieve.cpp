@@ -207,6 +207,28 @@ bool countAllOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6])
   return true;
 }
 
+/// countAllOnGpu without device allocation: the backend must exist already
+/// (creating it allocates), and the call's device memory is the code
+/// object's scratch array.
+bool countAllNoAllocOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6])
+{
+  std::fill(counts, counts + 6, 0);
+  psg::HipBackend* backend = existingHipBackend();
+  if (!backend)
+    return false;
+  if (start > stop)
+    return true;
+  addSmallPrimes(start, stop, counts);
+  if (stop >= 7)
+  {
+    uint64_t c[6];
+    backend->countAllNoAlloc(std::max<uint64_t>(start, 7), stop, c);
+    for (int i = 0; i < 6; i++)
+      counts[i] += c[i];
+  }
+  return true;
+}
+
 /// out[i] = count of type k_i - 1 in [start_i, stop_i].  mode 0 packs every
 /// query into the persistent engine, mode 2 does the same through a captured
 /// launch series replayed for every plan of the same shape, mode 1 is the

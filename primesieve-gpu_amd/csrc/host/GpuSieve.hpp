@@ -36,6 +36,9 @@ std::string& lastErrorRef();
 void resetBackend();
 /// Six counts of [start, stop] including 2/3/5; false when no GPU is usable.
 bool countAllOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6]);
+/// countAllOnGpu with no device allocation (the backend must exist);
+/// false when there is no backend yet.
+bool countAllNoAllocOnGpu(uint64_t start, uint64_t stop, uint64_t counts[6]);
 /// Batch counts (mode 0 persistent, 1 plain, 2 persistent through a captured
 /// launch series); false when no GPU is usable.
 bool countBatchOnGpu(const GpuQuery* q, size_t n, uint64_t* out, int mode);

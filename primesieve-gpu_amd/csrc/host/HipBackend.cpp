@@ -920,6 +920,19 @@ void HipBackend::countBatch(const std::vector<Interval>& iv, uint32_t outs, uint
   execute(iv, outs, out6, opt);
 }
 
+void HipBackend::countAllNoAlloc(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS])
+{
+  ExecOptions opt;
+  opt.runSegs = std::min(slabSegments_, segmentsPerLaunch_);
+  opt.slabSegs = opt.runSegs;
+  opt.windowSegs = windowSegs_;
+  opt.noAlloc = true;
+  // one slab's expected bucket entries take at most 40% of the scratch, so
+  // that its rooms (expected + margin) and the call's tables fit beside them
+  opt.entryBudget = std::min(opt.entryBudget, 0.4 * (double) scratchBytes_ / 4.0);
+  execute({ Interval{ lo, hi, 0 } }, 1, out, opt);
+}
+
 uint64_t HipBackend::count(uint64_t lo, uint64_t hi, int countType)
 {
   if (countType < 0 || countType >= PSG_NCOUNTS)

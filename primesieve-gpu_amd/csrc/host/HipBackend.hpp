@@ -117,6 +117,12 @@ public:
   /// captured one, replayed for every batch of the same shape.
   void countBatch(const std::vector<Interval>& iv, uint32_t outs, uint64_t* out6, bool graph = false);
 
+  /// countAll without any device allocation: the call's descriptors,
+  /// counters and bucket lists live in the code object's psg_scratch array
+  /// (PSG_SCRATCH_BYTES), and the sieving primes must already be cached.
+  /// Throws when the interval needs more scratch than there is.
+  void countAllNoAlloc(uint64_t lo, uint64_t hi, uint64_t out[PSG_NCOUNTS]);
+
   const psg_stats& lastStats() const { return stats_; }
   /// Make the device cache hold every sieving prime up to `limit` and copy
   /// the primes <= limit (up to cap; ascending) and their division magics

@@ -162,6 +162,25 @@ int psg_gpu_count_all(uint64_t start, uint64_t stop, uint64_t counts[6])
   }
 }
 
+int psg_gpu_count_all_noalloc(uint64_t start, uint64_t stop, uint64_t counts[6])
+{
+  if (!counts)
+    return -1;
+  std::lock_guard<std::mutex> lock(primesieve::gpu::gpuMutex());
+  try
+  {
+    if (!primesieve::gpu::countAllNoAllocOnGpu(start, stop, counts))
+      throw primesieve::primesieve_error("no GPU backend yet: a call that may allocate (psg_gpu_count_all, "
+                                         "psg_gpu_sieving_primes) creates it and its sieving-prime cache");
+    return 0;
+  }
+  catch (const std::exception& e)
+  {
+    primesieve::gpu::lastErrorRef() = e.what();
+    return -1;
+  }
+}
+
 int psg_gpu_count_batch(const psg_query* queries, size_t n, uint64_t* out, int mode)
 {
   if ((!queries || !out) && n)

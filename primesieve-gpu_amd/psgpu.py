@@ -117,6 +117,7 @@ def _declare(lib):
         "psg_gpu_worthwhile": (i32, [u64, u64]),
         "psg_gpu_try_count": (i32, [u64, u64, i32, ctypes.POINTER(u64)]),
         "psg_gpu_count_all": (i32, [u64, u64, ctypes.POINTER(u64)]),
+        "psg_gpu_count_all_noalloc": (i32, [u64, u64, ctypes.POINTER(u64)]),
         "psg_gpu_last_stats": (i32, [ctypes.POINTER(Stats)]),
         "psg_algorithmic_work": (i32, [u64, u64, ctypes.c_uint32, ctypes.POINTER(Work)]),
         "psg_measure_roofline": (i32, [ctypes.POINTER(Roofline)]),
@@ -242,6 +243,15 @@ def count_all(start, stop):
     """All six counts of [start, stop] from one sieve pass."""
     out = (ctypes.c_uint64 * 6)()
     if lib().psg_gpu_count_all(start, stop, out) != 0:
+        raise PrimesieveError(last_error() or "GPU count failed")
+    return list(out)
+
+
+def count_all_noalloc(start, stop):
+    """count_all with no device allocation (psg_gpu_count_all_noalloc): the
+    backend and its sieving-prime cache must exist already."""
+    out = (ctypes.c_uint64 * 6)()
+    if lib().psg_gpu_count_all_noalloc(start, stop, out) != 0:
         raise PrimesieveError(last_error() or "GPU count failed")
     return list(out)
 
