@@ -222,6 +222,25 @@ def workload_batch(m, model):
     return res
 
 
+def workload_batched_once(m):
+    """One warm persistent batch of the 8192 uneven intervals (for profilers:
+    the sieving-prime cache is built by an untimed call first)."""
+    q = np.load(os.path.join(ROOT, "tests", "golden", "config3_queries.npy"), allow_pickle=False)
+    qs = [(int(a), int(b), int(k)) for a, b, k in q]
+    m.count_all(10 ** 18, 10 ** 18 + 2 ** 33)
+    t0 = time.perf_counter()
+    got = m.count_batch(qs)
+    dt = time.perf_counter() - t0
+    exp = json.load(open(os.path.join(ROOT, "tests", "golden", "config3_expected.json")))
+    if any(got[int(i)] != v for i, v in exp.items()):
+        raise SystemExit("batched counts differ from the reference")
+    st = m.last_stats().as_dict()
+    return {"workload": "8192 uneven intervals, one warm persistent batch", "batched_s": dt,
+            "batched_intervals_per_s": len(qs) / dt,
+            "stats": {k: st[k] for k in ("kernel_ms", "fill_ms", "device_ms", "launches", "segments",
+                                         "bucket_entries")}}
+
+
 def workload_costmodel():
     path = os.path.join(PKG, "data", "costmodel_timings.npz")
     if not os.path.exists(path):
@@ -277,7 +296,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
-    ap.add_argument("--only", choices=["pi_1e10", "cost_model", "batched_intervals", "serving"],
+    ap.add_argument("--only", choices=["pi_1e10", "cost_model", "batched_intervals", "batched_once", "serving"],
                     help="run just this workload (no headline) and print its JSON, e.g. under a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -294,6 +313,7 @@ def main():
             model, cm = workload_costmodel()
         res = {"pi_1e10": lambda: workload_pi(m), "cost_model": lambda: cm,
                "batched_intervals": lambda: workload_batch(m, model),
+               "batched_once": lambda: workload_batched_once(m),
                "serving": lambda: workload_serving(m, model)}[args.only]()
         print(json.dumps({"workload": args.only, "result": res}), flush=True)
         return
