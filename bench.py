@@ -199,8 +199,25 @@ def workload_batch(m, model):
     res["batched_intervals_per_s"] = len(qs) / res["batched_s"]
     res["plain_intervals_per_s"] = len(qs) / res["plain_s"]
     res["speedup"] = res["plain_s"] / res["batched_s"]
+    # run-time grid search (the fallback tuner): segment bytes x waves x bucket
+    # threshold, timed on a seeded 512-query sample of the batch
+    import costmodel
+    t0 = time.perf_counter()
+    gt, gtimes = costmodel.grid_search_batch(qs, log=log)
+    tune_s = time.perf_counter() - t0
+    costmodel.apply_tuning(gt)
+    m.count_all(10 ** 18, 10 ** 18 + 2 ** 33)
+    t0 = time.perf_counter()
+    gb = m.count_batch(qs)
+    grid_s = time.perf_counter() - t0
+    m.set_tuning()
+    if gb != batched:
+        raise SystemExit("grid-tuned batch counts differ")
+    res["grid_search"] = {"choice": {"sieve_bytes": gt[0], "waves": gt[1], "bucket_factor": gt[2]},
+                          "tune_s": tune_s, "sample_queries": 512, "grid_points": len(gtimes),
+                          "batched_s": grid_s, "batched_intervals_per_s": len(qs) / grid_s,
+                          "sample_s_default": gtimes[costmodel.DEFAULT3], "sample_s_best": gtimes[gt]}
     if model is not None:
-        import costmodel
         t0 = time.perf_counter()
         (sb, waves), order = costmodel.choose_for_batch(
             model, [(a, b - a + 1, k) for a, b, k in qs])

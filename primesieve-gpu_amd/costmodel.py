@@ -10,9 +10,13 @@ count call from
 
 and is used to choose the tuning (segment bytes, waves) of an interval or of
 a whole batch, and to order a batch's queries by predicted cost.  Grid
-search over the same tuning grid is the fallback and the judge of its
-regret.  The model never touches a count: it only picks among geometries
-that all give bit-exact results.
+search is the fallback and the judge of its regret: over the same grid for
+the model's training and held-out timings, and at run time over a third
+axis as well, the bucket threshold (bucket_factor: primes above that many
+segments' bytes go to the HBM bucket lists), timed on a seeded sample of
+the batch itself -- the reference's tune() (tools/gpu_check.cpp:164-204)
+applied to a batch.  The model never touches a count: it only picks among
+geometries that all give bit-exact results.
 """
 import ctypes
 import json
@@ -28,6 +32,12 @@ SEG_BYTES = (32768, 65536, 131072, 163840)
 WAVES = (4, 8, 16)
 GRID = [(s, w) for s in SEG_BYTES for w in WAVES]
 DEFAULT = (131072, 16)          # the backend's own default geometry
+# the run-time grid's third axis: the bucket threshold in segments (only for
+# power-of-two segments; 160 KiB segments have no bucket tier)
+BUCKET_FACTORS = (4, 7, 12)
+GRID3 = [(s, w, b) for s in SEG_BYTES for w in WAVES
+         for b in (BUCKET_FACTORS if s & (s - 1) == 0 else (0,))]
+DEFAULT3 = (131072, 16, 7)
 
 
 def prime_count_estimate(x):
@@ -57,9 +67,62 @@ def draw_queries(n, seed, min_log2_len=16.0, max_log2_len=32.0):
     return out
 
 
-def collect(queries, grid=GRID, reps=1, log=None):
+def draw_sweep_queries(n, seed, min_log10_start=9.0, max_log10_start=15.0, min_log2_len=26.0,
+                       max_log2_len=37.0):
+    """Single large intervals in the manner of the [1e13, 1e13 + 1e11]
+    workload (six counts of one interval per call), swept over starts and
+    lengths: start = floor(10^u), u ~ U(9, 15); length = floor(2^v),
+    v ~ U(26, 37) (6.7e7 .. 1.4e11), rounded down to a multiple of 30;
+    k ~ U{1..6}."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n):
+        start = int(10.0 ** rng.uniform(min_log10_start, max_log10_start))
+        length = int(2.0 ** rng.uniform(min_log2_len, max_log2_len))
+        length -= length % 30
+        out.append((start, max(length, 30), int(rng.integers(1, 7))))
+    return out
+
+
+def apply_tuning(t):
+    """Make the backend use tuning t = (segment bytes, waves[, bucket factor])."""
+    sb, waves = t[0], t[1]
+    bf = t[2] if len(t) > 2 else 0
+    psgpu.set_tuning(sieve_bytes=sb, work_group_size=64 * waves, bucket_factor=bf)
+
+
+def grid_search_batch(queries, grid=GRID3, sample=512, seed=0, reps=2, log=None):
+    """Run-time grid search for a batch: time the persistent batch of a
+    seeded sample of `queries` (start, stop, k) under every tuning of `grid`
+    (each a new backend; its sieving-prime cache is filled by an untimed call
+    first) and return (fastest tuning, {tuning: seconds}).  The backend is
+    left with the default tuning."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    idx = np.sort(rng.choice(len(queries), size=min(sample, len(queries)), replace=False))
+    qs = [queries[i] for i in idx]
+    top = max(b for _, b, _ in qs)
+    times = {}
+    try:
+        for t in grid:
+            apply_tuning(t)
+            psgpu.count_all(max(top - 29, 0), top)
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                psgpu.count_batch(qs)
+                best = min(best, time.perf_counter() - t0)
+            times[t] = best
+            if log:
+                log("grid %s: %.4f s" % (t, best))
+    finally:
+        psgpu.set_tuning()
+    return min(times, key=times.get), times
+
+
+def collect(queries, grid=GRID, reps=1, log=None, done=None):
     """Time one count call per (query, tuning) on the GPU.  Returns an
-    (n_queries, len(grid)) array of device milliseconds (min over reps)."""
+    (n_queries, len(grid)) array of device milliseconds (min over reps).
+    done(times, gi) is called after each tuning (for partial saves)."""
     times = np.zeros((len(queries), len(grid)), dtype=np.float64)
     top = max(queries, key=lambda q: q[0] + q[1])
     for gi, (sb, waves) in enumerate(grid):
@@ -78,6 +141,8 @@ def collect(queries, grid=GRID, reps=1, log=None):
                 log("config %s: %d queries, %.1f s" % ((sb, waves), qi + 1, time.perf_counter() - t0))
         if log:
             log("config %s: %d queries in %.1f s" % ((sb, waves), len(queries), time.perf_counter() - t0))
+        if done:
+            done(times, gi)
     psgpu.set_tuning()
     return times
 

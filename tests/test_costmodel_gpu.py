@@ -98,3 +98,26 @@ def test_tuner_regret_on_held_out_queries(cm):
     print("regret", r, flush=True)
     assert r["total_regret"] < r["default_total_regret"]
     assert r["gain_over_default"] > 1.0
+
+
+def test_runtime_grid_search_on_a_batch(cm, gpu, golden):
+    """The fallback tuner times a seeded sample of a batch under each tuning
+    of a (segment bytes, waves, bucket threshold) grid and returns the
+    fastest; a batch run under the chosen tuning keeps the reference's
+    counts."""
+    q = np.load(os.path.join(ROOT, "tests", "golden", "config3_queries.npy"), allow_pickle=False)
+    exp = golden("config3_expected.json")
+    idx = sorted(int(i) for i in exp)[:96]
+    qs = [(int(q[i, 0]), int(q[i, 1]), int(q[i, 2])) for i in idx]
+    grid = [(131072, 16, 7), (131072, 16, 4), (65536, 8, 12), (163840, 16, 0)]
+    best, times = cm.grid_search_batch(qs, grid=grid, sample=48, seed=3, reps=1)
+    print("grid search", best, times, flush=True)
+    assert best in grid and set(times) == set(grid) and all(t > 0 for t in times.values())
+    cm.apply_tuning(best)
+    try:
+        t = gpu.get_tuning()
+        # segment bytes are rounded down to what fits in LDS beside the segment's tail
+        assert t["work_group_size"] == 64 * best[1] and best[0] - 4096 < t["sieve_bytes"] <= best[0]
+        assert gpu.count_batch(qs) == [exp[str(i)] for i in idx]
+    finally:
+        gpu.set_tuning()
