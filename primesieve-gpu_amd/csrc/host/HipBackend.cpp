@@ -379,8 +379,8 @@ uint32_t HipBackend::fillGrid(uint32_t nItems)
   if (!fillWGs_)
   {
     int perCU = 0;
-    const uint32_t lds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
-    if (api_->ModuleOccupancy(&perCU, fnBucketFill_, (int) PSG_BUCKET_THREADS, lds) != hipSuccess || perCU < 1)
+    if (api_->ModuleOccupancy(&perCU, fnBucketFill_, (int) PSG_BUCKET_THREADS, PSG_BUCKET_LDS) != hipSuccess
+        || perCU < 1)
       perCU = 1;
     fillWGs_ = (uint32_t) perCU * (dev_.computeUnits ? dev_.computeUnits : 256);
   }
@@ -729,7 +729,7 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
            "hipMemcpyAsync descriptors");
 
   // ---- launches -----------------------------------------------------------------
-  const uint32_t fillLds = (PSG_BUCKET_STAGE + 3u * PSG_MAX_WINDOW_SEGS) * 4u;
+  const uint32_t fillLds = PSG_BUCKET_LDS;
   const bool fixedGrids = opt.graph;
   const auto recordAddr = [&](size_t si) { return desc + PSG_DESC_HEADER + si * PSG_SLAB_RECORD; };
   const auto launchClear = [&](hipStream_t st) {

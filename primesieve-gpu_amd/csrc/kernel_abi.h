@@ -99,6 +99,13 @@ typedef struct PsgSieveArgs {
 #define PSG_BUCKET_HITS 32u          /* hits a fill thread stages per stage */
 #define PSG_BUCKET_STAGE (PSG_BUCKET_THREADS * PSG_BUCKET_HITS)
 #define PSG_MAX_WINDOW_SEGS PSG_BUCKET_THREADS   /* segments per bucket window */
+/* The fill's per-segment histogram has several copies, interleaved per
+ * segment (word seg * C + copy, copy = thread mod C), so that lanes counting
+ * hits of one segment rarely hit the same LDS word: C = 4 for windows of up
+ * to 128 segments, 2 up to 256. */
+#define PSG_BUCKET_HIST_WORDS (2u * PSG_MAX_WINDOW_SEGS)
+/* dynamic LDS of psg_bucket_fill: stage, histogram copies, per-segment placement (u64) */
+#define PSG_BUCKET_LDS ((PSG_BUCKET_STAGE + PSG_BUCKET_HIST_WORDS + 2u * PSG_MAX_WINDOW_SEGS) * 4u)
 
 /* One bucket window (32 bytes): consecutive segments of one run walked by
  * nChunks workgroups, each over an interleaved share of the big primes

@@ -1203,9 +1203,10 @@ psg_bucket_fill(uint64_t argsAddr)
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = PSG_BUCKET_THREADS;        // the launch's block size
   uint32_t* stage = lds;                               // [kStageF][nthreads], then sorted
-  uint32_t* hist = stage + PSG_BUCKET_STAGE;           // [W] counts, then scatter cursors
+  // [W][C] counts per segment and copy (copy = thread mod C), then scatter cursors
+  uint32_t* hist = stage + PSG_BUCKET_STAGE;
   // [W] first sorted slot | fit << 16 | entry index of the segment's run << 32
-  uint64_t* place = (uint64_t*) (hist + PSG_MAX_WINDOW_SEGS);
+  uint64_t* place = (uint64_t*) (hist + PSG_BUCKET_HIST_WORDS);
   __shared__ __align__(16) uint32_t ctab[240];
   __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
   __shared__ uint32_t sItem;
@@ -1228,7 +1229,11 @@ psg_bucket_fill(uint64_t argsAddr)
     const uint32_t W = w.segs;                           // <= PSG_MAX_WINDOW_SEGS <= blockDim
     const auto* __restrict__ listOff = gptr<uint32_t>(a.listOff) + w.firstSeg;
     uint32_t* listCount = (uint32_t*) a.listCount + w.firstSeg;
-    for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
+    // C = 2^cs histogram copies per segment (uniform over the item)
+    const uint32_t cs = (W <= PSG_BUCKET_HIST_WORDS / 4u) ? 2u : 1u;
+    const uint32_t copy = tid & ((1u << cs) - 1u);
+    const uint32_t HW = W << cs;
+    for (uint32_t i = tid; i < HW; i += nthreads) hist[i] = 0;
 
     BucketWalk st;
     st.k = a.bigBegin + c * nthreads + tid;
@@ -1240,12 +1245,22 @@ psg_bucket_fill(uint64_t argsAddr)
       const uint32_t n = bucket_walk_stage(a, w, st, stride, ctab, kStageF,
           [&](uint32_t i, uint32_t seg, uint32_t hit) {
             stage[i * nthreads + tid] = hit;
-            atomicAdd(&hist[seg], 1u);
+            atomicAdd(&hist[(seg << cs) | copy], 1u);
           });
       const int more = __syncthreads_or((st.p != 0u || st.pn != 0u) ? 1 : 0);
 
-      // exclusive prefix of the counts; thread i owns segment i
-      const uint32_t cnt = (tid < W) ? hist[tid] : 0u;
+      // exclusive prefix of the counts; thread i owns segment i and its copies
+      uint32_t cc[4] = {0u, 0u, 0u, 0u};
+      if (tid < W) {
+        if (cs == 2u) {
+          const uint4 h4 = *(const uint4*) &hist[tid << 2];
+          cc[0] = h4.x; cc[1] = h4.y; cc[2] = h4.z; cc[3] = h4.w;
+        } else {
+          const uint2 h2 = *(const uint2*) &hist[tid << 1];
+          cc[0] = h2.x; cc[1] = h2.y;
+        }
+      }
+      const uint32_t cnt = cc[0] + cc[1] + cc[2] + cc[3];
       uint32_t x = cnt;
       for (uint32_t d = 1; d < 64u; d <<= 1) {
         const uint32_t y = (uint32_t) __shfl_up((int) x, d);
@@ -1262,7 +1277,11 @@ psg_bucket_fill(uint64_t argsAddr)
       const uint32_t f = before + x - cnt;
       uint32_t at = 0;
       if (tid < W) {
-        hist[tid] = f;
+        // each copy's cursor starts after the segment's earlier copies
+        if (cs == 2u)
+          *(uint4*) &hist[tid << 2] = make_uint4(f, f + cc[0], f + cc[0] + cc[1], f + cc[0] + cc[1] + cc[2]);
+        else
+          *(uint2*) &hist[tid << 1] = make_uint2(f, f + cc[0]);
         if (cnt) at = atomicAdd(&listCount[tid], cnt);   // consumed after the scatter
       }
       uint32_t e[kStageF];
@@ -1277,7 +1296,7 @@ psg_bucket_fill(uint64_t argsAddr)
         uint32_t slot[8];
 #pragma unroll
         for (uint32_t q = 0; q < 8; q++)
-          slot[q] = (g + q < n) ? atomicAdd(&hist[e[g + q] >> eshift], 1u) : 0u;
+          slot[q] = (g + q < n) ? atomicAdd(&hist[((e[g + q] >> eshift) << cs) | copy], 1u) : 0u;
 #pragma unroll
         for (uint32_t q = 0; q < 8; q++)
           if (g + q < n) stage[slot[q]] = e[g + q];
@@ -1311,7 +1330,7 @@ psg_bucket_fill(uint64_t argsAddr)
         }
       }
       appended += total;
-      for (uint32_t i = tid; i < W; i += nthreads) hist[i] = 0;
+      for (uint32_t i = tid; i < HW; i += nthreads) hist[i] = 0;
       __syncthreads();
       if (!more) break;
     }
