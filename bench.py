@@ -348,6 +348,9 @@ def main():
                 "launches_per_step": launches, "lds_bytes_per_launch": lds_bytes_launch,
                 "crossings": work.crossings, "words32": work.words32,
                 "peak_source": "measured ds_or_b32 microbenchmark (psg_lds_and mode 2, conflict-free)",
+                # the same strike with random word addresses, the access pattern of a scattered sieve
+                "peak_random_address": lds["or_random_gbs"],
+                "frac_vs_random_address_peak": achieved / lds["or_random_gbs"],
                 "measured": lds}
     out = {
         "metric": "integers sieved/s on 1 MI355X, exact counts",

@@ -12,9 +12,9 @@ Two kinds of data end up here:
 The config-3 query list (config3_queries.npy) is drawn from a fixed seed and
 frozen as data so that expected counts never depend on floating-point pow().
 
-Usage: python tests/golden/make_golden.py [extra]   (needs oracle/_ref built: make -C oracle ref;
+Usage: python tests/golden/make_golden.py [extra|wide]   (needs oracle/_ref built: make -C oracle ref;
        "extra" writes only cases_high.json, config3_decades.json and
-       config5_expected.json)
+       config5_expected.json; "wide" only cases_wide.json)
 """
 import ctypes
 import json
@@ -140,6 +140,21 @@ def high_cases(fns):
             for a, b in spans]
 
 
+def wide_cases():
+    """Six counts of 1e11-wide intervals at 1e15 and 1e17 (several bucket slabs
+    per call: their expected bucket entries exceed one slab's budget), from the
+    reference's one-pass ParallelSieve (oracle/ref_count6.cpp)."""
+    lib = ctypes.CDLL(REF_LIB)
+    lib.psg_ref_count6.restype = None
+    lib.psg_ref_count6.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
+    out = []
+    for a in (10 ** 15, 10 ** 17):
+        c = (ctypes.c_uint64 * 6)()
+        lib.psg_ref_count6(a, a + 10 ** 11, c, os.cpu_count() or 1)
+        out.append({"start": a, "stop": a + 10 ** 11, "counts": {str(t): c[t] for t in range(6)}})
+    return out
+
+
 def config3_decade_subset(fns, q, per_decade=16, skip=()):
     """Reference counts for the first `per_decade` config-3 queries of every
     start decade 10^9 .. 10^17 (lengths up to 2^32), beyond those in `skip`."""
@@ -173,6 +188,11 @@ def main():
     if not os.path.exists(REF_LIB):
         sys.exit("build the reference library first: make -C oracle ref")
     fns = ref_lib()
+    if len(sys.argv) > 1 and sys.argv[1] == "wide":
+        with open(os.path.join(HERE, "cases_wide.json"), "w") as f:
+            json.dump(wide_cases(), f)
+        print("wrote cases_wide.json", flush=True)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "extra":
         # the fixtures added later, written without rewriting the others
         q = np.load(os.path.join(HERE, "config3_queries.npy"), allow_pickle=False)

@@ -106,6 +106,18 @@ def test_high_ranges_golden(gpu, golden):
         assert gpu.count(top["start"], top["stop"], t) == top["counts"][str(t)]
 
 
+def test_wide_intervals_over_several_bucket_slabs(gpu, golden):
+    """1e11-wide intervals at 1e15 and 1e17 expect 6.2e9 and 9.5e9 bucket
+    entries, more than one slab's budget: each call is cut into several runs
+    (and slabs) and must still give the reference's six counts."""
+    for c in golden("cases_wide.json"):
+        got = gpu.count_all(c["start"], c["stop"])
+        st = gpu.last_stats()
+        _say("wide", c["start"], st.as_dict())
+        assert got == [c["counts"][str(t)] for t in range(6)], (c["start"], got)
+        assert st.launches >= 2 and st.bucket_retries == 0
+
+
 def test_work_group_size_power_of_two(gpu, oracle):
     """Workgroup sizes are rounded down to a power of two, as the reference
     requires (src/gpu/GpuBackend.hpp:49-50, src/gpu/cuda/CudaBackend.cpp:260-263);
