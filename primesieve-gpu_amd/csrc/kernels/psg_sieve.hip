@@ -938,6 +938,10 @@ __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
   return (uint32_t) ((r < 20u ? (kBitLo >> (3u * r)) : (kBitHi >> (3u * (r - 20u)))) & 7u);
 }
 
+// Primes a thread has in flight ahead of the one it walks: a visit is often
+// shorter than the latency of a prime's load.
+constexpr uint32_t kPrefetch = 4;
+
 // Resumable walk of one thread over its share of a window's big primes.  The
 // big primes are cut into tiles of blockDim consecutive primes and the tiles
 // are dealt round-robin to the window's chunks (tile m*nChunks + c), so that
@@ -950,8 +954,8 @@ __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
 //     eight class starts.
 // A stage stops after `limit` hits; the walk resumes there in the next stage.
 struct BucketWalk {
-  uint32_t k;        // index of the prefetched prime
-  uint32_t pn;       // the prefetched prime primes[k] (0: none left)
+  uint32_t k;        // index of the last prefetched prime
+  uint32_t pn[kPrefetch];   // the next primes, in order (0: none left)
   uint32_t p;        // prime being walked (0: none)
   uint32_t R;        // window base mod p
   uint32_t j;        // dense: current class; 8 marks a sparse prime
@@ -982,14 +986,16 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
     // multiple or class start) then runs once per prime for the whole wave
     // instead of interleaving with the hit loops of other lanes.
     if (__all(s.p == 0u)) {
-      if (__all(s.pn == 0u)) break;
-      if (s.pn == 0u) continue;               // this lane is done; others take primes
-      // take the prefetched prime and fetch the next one while this one is walked
-      const uint32_t p = s.pn;
+      if (__all(s.pn[0] == 0u)) break;
+      if (s.pn[0] == 0u) continue;            // this lane is done; others take primes
+      // take the first prefetched prime and fetch the one kPrefetch ahead
+      const uint32_t p = s.pn[0];
       const double rp = recip(p);
       const uint32_t R = mod_big(w.base, baseF, p, rp);    // bucket primes are > 2^16
+#pragma unroll
+      for (uint32_t i = 0; i + 1 < kPrefetch; i++) s.pn[i] = s.pn[i + 1];
       s.k += stride;
-      s.pn = (s.k < w.bigEnd) ? primes[s.k] : 0u;
+      s.pn[kPrefetch - 1] = (s.k < w.bigEnd) ? primes[s.k] : 0u;
       s.p = p;
       s.R = R;
       if ((uint64_t) p * 8u > V) {
@@ -1237,7 +1243,11 @@ psg_bucket_fill(uint64_t argsAddr)
 
     BucketWalk st;
     st.k = a.bigBegin + c * nthreads + tid;
-    st.pn = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kPrefetch; i++) {
+      if (i) st.k += nthreads * w.nChunks;
+      st.pn[i] = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
+    }
     st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
     const uint32_t stride = nthreads * w.nChunks;
     __syncthreads();
@@ -1247,7 +1257,7 @@ psg_bucket_fill(uint64_t argsAddr)
             stage[i * nthreads + tid] = hit;
             atomicAdd(&hist[(seg << cs) | copy], 1u);
           });
-      const int more = __syncthreads_or((st.p != 0u || st.pn != 0u) ? 1 : 0);
+      const int more = __syncthreads_or((st.p != 0u || st.pn[0] != 0u) ? 1 : 0);
 
       // exclusive prefix of the counts; thread i owns segment i and its copies
       uint32_t cc[4] = {0u, 0u, 0u, 0u};
