@@ -866,14 +866,20 @@ __device__ __forceinline__ void count_and_fill(const PsgSieveArgs& args, const P
     const uint64_t t1 = x & (x >> 1);
     const uint64_t t2 = t1 & (x >> 2);
     const uint64_t t3 = t2 & (x >> 3);
-    const uint64_t t4 = t3 & (x >> 4);
-    const uint64_t t5 = t4 & (x >> 5);
+    const uint64_t quad = t3 & 0x0202020202020202ull;   // bits 1..4
     c[0] += __popcll(x);
     c[1] += __popcll(t1 & 0x4a4a4a4a4a4a4a4aull);   // (11,13) (17,19) (29,31)
     c[2] += __popcll(t2 & 0x0f0f0f0f0f0f0f0full);   // 3 consecutive bits from 0..3
-    c[3] += __popcll(t3 & 0x0202020202020202ull);   // bits 1..4
-    c[4] += __popcll(t4 & 0x0303030303030303ull);   // bits 0..4, 1..5
-    c[5] += __popcll(t5 & 0x0101010101010101ull);   // bits 0..5
+    // Every quintuplet and sextuplet pattern contains the quadruplet bits
+    // 1..4 of its byte, and quadruplets are rare: the last three counts are
+    // skipped (per wave, when no lane has one) for words without one.
+    if (quad) {
+      const uint64_t t4 = t3 & (x >> 4);
+      const uint64_t t5 = t4 & (x >> 5);
+      c[3] += __popcll(quad);
+      c[4] += __popcll(t4 & 0x0303030303030303ull);   // bits 0..4, 1..5
+      c[5] += __popcll(t5 & 0x0101010101010101ull);   // bits 0..5
+    }
   };
   // the whole segment inside the interval: no edge masks
   const bool inside = B > run.gLo && B + S <= run.gHi;
