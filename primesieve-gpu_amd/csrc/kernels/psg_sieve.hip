@@ -261,7 +261,9 @@ __device__ __forceinline__ void strike_few(uint32_t p, uint32_t a, uint32_t R, u
       i += p;                                   // skip the prime itself
     uint32_t m = 1u << (((i & 3u) << 3) + c);
     for (uint32_t k = 0; k < nh; k++) {
-      if (i < S) lds_or(lds_word(i), m);
+      // the first hit i < p <= S is inside the segment unless it was moved
+      // past the prime itself
+      if ((!kNear && k == 0) || i < S) lds_or(lds_word(i), m);
       i += p;
       m = rotl32(m, rot);
     }
