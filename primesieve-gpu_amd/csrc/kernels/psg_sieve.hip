@@ -46,6 +46,7 @@
 // the same word at once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "../kernel_abi.h"
 
@@ -1308,17 +1309,23 @@ psg_bucket_fill(uint64_t argsAddr)
       __syncthreads();                                   // cursors set, stage read
 
       // Scatter by segment, eight hits at a time: their cursor atomics first,
-      // then their stores, so that eight atomics are in flight at once.
+      // then their stores, so that eight atomics are in flight at once.  A
+      // wave whose threads all staged a full stage (every stage of an item
+      // but its last) skips the per-hit test.
+      const auto scatter = [&](auto full) {
 #pragma unroll
-      for (uint32_t g = 0; g < kStageF; g += 8) {
-        uint32_t slot[8];
+        for (uint32_t g = 0; g < kStageF; g += 8) {
+          uint32_t slot[8];
 #pragma unroll
-        for (uint32_t q = 0; q < 8; q++)
-          slot[q] = (g + q < n) ? atomicAdd(&hist[((e[g + q] >> eshift) << cs) | copy], 1u) : 0u;
+          for (uint32_t q = 0; q < 8; q++)
+            slot[q] = (full.value || g + q < n) ? atomicAdd(&hist[((e[g + q] >> eshift) << cs) | copy], 1u) : 0u;
 #pragma unroll
-        for (uint32_t q = 0; q < 8; q++)
-          if (g + q < n) stage[slot[q]] = e[g + q];
-      }
+          for (uint32_t q = 0; q < 8; q++)
+            if (full.value || g + q < n) stage[slot[q]] = e[g + q];
+        }
+      };
+      if (__all(n == kStageF)) scatter(std::true_type{});
+      else scatter(std::false_type{});
       if (tid < W) {
         const uint32_t room = listOff[tid + 1u] - listOff[tid];
         if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
