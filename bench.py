@@ -219,8 +219,7 @@ def workload_batch(m, model):
                           "sample_s_default": gtimes[costmodel.DEFAULT3], "sample_s_best": gtimes[gt]}
     if model is not None:
         t0 = time.perf_counter()
-        (sb, waves), order = costmodel.choose_for_batch(
-            model, [(a, b - a + 1, k) for a, b, k in qs])
+        (sb, waves), order, confirm = costmodel.plan_batch(model, qs)
         m.set_tuning(sieve_bytes=sb, work_group_size=64 * waves)
         m.count_all(10 ** 18, 10 ** 18 + 10 ** 6)
         plan_s = time.perf_counter() - t0
@@ -233,7 +232,8 @@ def workload_batch(m, model):
             back[i] = tuned[j]
         if back != batched:
             raise SystemExit("model-packed counts differ")
-        res["model_choice"] = {"sieve_bytes": sb, "waves": waves, "plan_s": plan_s}
+        res["model_choice"] = {"sieve_bytes": sb, "waves": waves, "plan_s": plan_s,
+                               "confirmed_on_sample": {"%d/%d/%d" % t: s for t, s in confirm.items()}}
         res["model_batched_intervals_per_s"] = len(qs) / res["model_batched_s"]
         res["model_speedup_vs_plain"] = res["plain_s"] / res["model_batched_s"]
     return res

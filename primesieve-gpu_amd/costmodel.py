@@ -282,6 +282,26 @@ def choose_for_batch(model, queries, grid=GRID, min_gain=0.05):
     return grid[g], order
 
 
+def plan_batch(model, queries, sample=512, seed=0, reps=2, min_gain=0.05):
+    """The tuning and packing order of a batch of (start, stop, k).
+
+    The model proposes (choose_for_batch).  It is trained on single large
+    intervals, while a batch of short ones spends its time on each query's
+    sweep over the sieving primes, so a proposal other than the default
+    geometry is confirmed by measurement: both are timed on a seeded sample
+    of the batch (grid_search_batch over the two) and the faster is kept.
+    Returns ((segment bytes, waves), order, {tuning: sample seconds} or {})."""
+    (sb, waves), order = choose_for_batch(model, [(a, b - a + 1, k) for a, b, k in queries],
+                                          min_gain=min_gain)
+    pick = (sb, waves, DEFAULT3[2])
+    times = {}
+    if pick != DEFAULT3:
+        best, times = grid_search_batch(queries, grid=[DEFAULT3, pick], sample=sample, seed=seed,
+                                        reps=reps)
+        sb, waves = best[0], best[1]
+    return (sb, waves), order, times
+
+
 def regret(chosen, times, default_index):
     """Regret of choosing `chosen` (per query) against grid search on the
     measured times (n x grid), and the gain over the fixed default."""

@@ -121,3 +121,30 @@ def test_runtime_grid_search_on_a_batch(cm, gpu, golden):
         assert gpu.count_batch(qs) == [exp[str(i)] for i in idx]
     finally:
         gpu.set_tuning()
+
+
+def test_plan_batch_confirms_the_model_by_timing(cm, gpu, golden):
+    """The model's batch tuning is kept only when timing a sample of the
+    batch confirms it against the default geometry; the packing order is a
+    permutation and the packed batch keeps the reference's counts."""
+    q = np.load(os.path.join(ROOT, "tests", "golden", "config3_queries.npy"), allow_pickle=False)
+    exp = golden("config3_expected.json")
+    idx = sorted(int(i) for i in exp)[:64]
+    qs = [(int(q[i, 0]), int(q[i, 1]), int(q[i, 2])) for i in idx]
+    net = cm.MLP(seed=4)
+    proposal, _ = cm.choose_for_batch(net, [(a, b - a + 1, k) for a, b, k in qs])
+    (sb, waves), order, times = cm.plan_batch(net, qs, sample=32, seed=1, reps=1)
+    print("proposal", proposal, "chosen", (sb, waves), times, flush=True)
+    assert sorted(order.tolist()) == list(range(len(qs)))
+    if proposal == cm.DEFAULT:
+        assert (sb, waves) == cm.DEFAULT and times == {}
+    else:
+        pick = proposal + (cm.DEFAULT3[2],)
+        assert set(times) == {cm.DEFAULT3, pick}
+        assert (sb, waves) == min(times, key=times.get)[:2]
+    gpu.set_tuning(sieve_bytes=sb, work_group_size=64 * waves)
+    try:
+        got = gpu.count_batch([qs[i] for i in order])
+        assert [got[j] for j in np.argsort(order)] == [exp[str(i)] for i in idx]
+    finally:
+        gpu.set_tuning()
