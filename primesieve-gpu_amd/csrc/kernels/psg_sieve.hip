@@ -410,6 +410,22 @@ __device__ __forceinline__ void init_ctab(uint32_t* ctab)
   }
 }
 
+// Window index (B + 16 tid) mod P of each tile for the 16-byte run of thread
+// tid of the segment at B.  B is the same for the whole workgroup, so its
+// remainders are taken once, in scalar registers.
+__device__ __forceinline__ void tile_start(uint64_t B, uint32_t tid, uint32_t (&idx)[PSG_NTILES])
+{
+  constexpr uint32_t kPeriod[PSG_NTILES] = PSG_TILE_PERIODS;
+  const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) (B >> 32));
+  const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) B);
+  const uint64_t Bu = ((uint64_t) hi << 32) | lo;
+#pragma unroll
+  for (int t = 0; t < PSG_NTILES; t++) {
+    const uint32_t x = (uint32_t) (Bu % kPeriod[t]) + (16u * tid) % kPeriod[t];
+    idx[t] = (x >= kPeriod[t]) ? x - kPeriod[t] : x;
+  }
+}
+
 // Fill the segment from the pre-sieve tiles (stored complemented: a set bit
 // marks a multiple of a tile prime): word w is the OR over the tiles of
 // window (B + 4w) mod P.  Tile 0 comes from global memory, the small
@@ -428,11 +444,10 @@ __device__ __forceinline__ void fill_tiles(uint32_t* sieve, uint32_t words, uint
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
   uint32_t idx[PSG_NTILES], stp[PSG_NTILES];
+  tile_start(B, tid, idx);
 #pragma unroll
-  for (int t = 0; t < PSG_NTILES; t++) {
-    idx[t] = (uint32_t) ((B + 16ull * tid) % kPeriod[t]);
+  for (int t = 0; t < PSG_NTILES; t++)
     stp[t] = (16u * nthreads) % kPeriod[t];
-  }
   const auto run = [&]() {
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
@@ -842,12 +857,11 @@ __device__ __forceinline__ void count_and_fill(const PsgSieveArgs& args, const P
   const auto* __restrict__ tiles = gptr<uint32_t>(args.tiles);
   const bool fill = hasNext && tiles;              // else the next segment starts from zeros
   const bool lds = args.tilesInLds != 0;
-  uint32_t idx[PSG_NTILES], stp[PSG_NTILES];
+  uint32_t idx[PSG_NTILES] = {0u, 0u, 0u, 0u, 0u}, stp[PSG_NTILES];
+  if (fill) tile_start(nB, tid, idx);
 #pragma unroll
-  for (int t = 0; t < PSG_NTILES; t++) {
-    idx[t] = fill ? (uint32_t) ((nB + 16ull * tid) % kPeriod[t]) : 0u;
+  for (int t = 0; t < PSG_NTILES; t++)
     stp[t] = (16u * nthreads) % kPeriod[t];
-  }
   const auto next_run = [&]() {                    // as in fill_tiles
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
