@@ -247,11 +247,13 @@ __device__ __forceinline__ void strike_medium(uint32_t p, uint32_t a, uint32_t R
 
 // strike_medium for S/4 < p <= S, where a class has at most nh <= 4 hits
 // (nh is uniform over the wave): the hits of a class one after another, the
-// word mask rotated by 8 * (p mod 4) bits from one hit to the next.
-template <bool kNear>
+// word mask rotated by 8 * (p mod 4) bits from one hit to the next.  kNh > 0
+// fixes nh at compile time (the loop unrolls).
+template <bool kNear, uint32_t kNh = 0>
 __device__ __forceinline__ void strike_few(uint32_t p, uint32_t a, uint32_t R, uint64_t B,
                                            uint32_t S, const ClassRow& row, uint32_t nh)
 {
+  if (kNh) nh = kNh;
   const uint32_t a1 = a | 0x10000u, nR = ~R;                       // p <= S: a < 2^16
   const uint32_t rot = (p & 3u) << 3;
 #pragma unroll
@@ -261,7 +263,8 @@ __device__ __forceinline__ void strike_few(uint32_t p, uint32_t a, uint32_t R, u
     if (kNear && (ent & 0xffu) == 1u && B + i < (uint64_t) p)
       i += p;                                   // skip the prime itself
     uint32_t m = 1u << (((i & 3u) << 3) + c);
-    for (uint32_t k = 0; k < nh; k++) {
+#pragma unroll
+    for (uint32_t k = 0; k < (kNh ? kNh : nh); k++) {
       // the first hit i < p <= S is inside the segment unless it was moved
       // past the prime itself
       if ((!kNear && k == 0) || i < S) lds_or(lds_word(i), m);
@@ -280,6 +283,7 @@ __device__ __forceinline__ void strike_tier2(uint32_t p, uint32_t a, uint32_t R,
   if (4u * pMin > S) {
     const uint32_t nh = (2u * pMin > S) ? 2u : (3u * pMin > S) ? 3u : 4u;
     if (nearSmall) strike_few<true>(p, a, R, B, S, row, nh);
+    else if (nh == 2u) strike_few<false, 2>(p, a, R, B, S, row, nh);   // p > S/2: the most primes
     else strike_few<false>(p, a, R, B, S, row, nh);
   } else {
     if (nearSmall) strike_medium<true>(p, a, R, B, S, row);
