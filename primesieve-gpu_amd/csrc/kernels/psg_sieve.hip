@@ -284,7 +284,8 @@ __device__ __forceinline__ void strike_tier2(uint32_t p, uint32_t a, uint32_t R,
     const uint32_t nh = (2u * pMin > S) ? 2u : (3u * pMin > S) ? 3u : 4u;
     if (nearSmall) strike_few<true>(p, a, R, B, S, row, nh);
     else if (nh == 2u) strike_few<false, 2>(p, a, R, B, S, row, nh);   // p > S/2: the most primes
-    else strike_few<false>(p, a, R, B, S, row, nh);
+    else if (nh == 3u) strike_few<false, 3>(p, a, R, B, S, row, nh);
+    else strike_few<false, 4>(p, a, R, B, S, row, nh);
   } else {
     if (nearSmall) strike_medium<true>(p, a, R, B, S, row);
     else strike_medium<false>(p, a, R, B, S, row);
