@@ -78,6 +78,9 @@ __device__ __forceinline__ T load_args(uint64_t addr)
 }
 
 __device__ __constant__ uint32_t kOffsets[8] = {7, 11, 13, 17, 19, 23, 29, 31};
+// the residues coprime to 30, ascending, and as a bit set (bit q for q = 0..29)
+__device__ __constant__ uint32_t kCoprimes[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+constexpr uint32_t kCoprime30 = 0x208A2882u;
 
 // R = B mod p for B < 2^63 using M = ceil(2^(64+L)/p):
 // floor(B/p) = mulhi(B, M) >> L.  Only the low 32 bits of B - q*p matter.
@@ -86,6 +89,15 @@ __device__ __forceinline__ uint32_t mod_magic(uint64_t B, uint32_t p, uint64_t M
   const uint32_t L = 31u - (uint32_t) __clz((int) p);
   const uint64_t q = __umul64hi(B, M) >> L;
   return (uint32_t) B - (uint32_t) q * p;
+}
+
+// floor(y / p) for y < 2^32 with the same magic: the high 64 bits of y * M
+// (y * M_hi plus the carry word of y * M_lo), shifted by L.
+__device__ __forceinline__ uint32_t div_magic32(uint32_t y, uint32_t p, uint64_t M)
+{
+  const uint32_t L = 31u - (uint32_t) __clz((int) (p | 1u));
+  const uint64_t hi = (uint64_t) y * (uint32_t) (M >> 32) + __umulhi(y, (uint32_t) M);
+  return (uint32_t) (hi >> 32) >> L;
 }
 
 __device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
@@ -336,6 +348,44 @@ __device__ __forceinline__ void strike_big_lane(const uint32_t (&p)[kN], const u
     if (p[m]) strike_big<kNear, kSmall>(p[m], a[m], R[m], B, S, row[m]);
 }
 
+// The hits of the kN primes of a lane (p[m] = 0: none) with 8p > 30S and
+// p < 30 * 2^16 in a segment at B >= p (R = B mod p, M the division magic).
+// The segment's 30S values hold at most 8 consecutive multiples of such a
+// prime, so at most 3 whose cofactor is coprime to 30.  With B = Qp + R, the
+// first multiple at or after the segment's first value 30B + 7 is
+// p * (30Q + t), t = ceil((30R + 7) / p) in [1, 30], so its hits are among
+// the three classes of its cofactor-ordered row from the first coprime
+// residue >= t mod 30; each is tested exactly, as in strike_big.  Three
+// table reads and three strikes per prime instead of two row reads and
+// eight strikes, and no class offsets for the other five classes.
+template <uint32_t kN>
+__device__ __forceinline__ void strike_three_lane(const uint32_t (&p)[kN], const uint32_t (&R)[kN],
+                                                  const uint64_t (&M)[kN], uint32_t S, const uint2* ctabq)
+{
+  uint32_t a[kN];
+  uint2 e[kN][3];
+#pragma unroll
+  for (uint32_t m = 0; m < kN; m++) {
+    a[m] = p[m] / 30u;
+    const uint32_t b = p[m] - a[m] * 30u;
+    const uint32_t row = __popc(kCoprime30 & ((1u << b) - 1u));          // rank of b
+    const uint32_t t = div_magic32(30u * R[m] + 6u + p[m], p[m], M[m]);  // ceil((30R + 7) / p)
+    const uint32_t s0 = __popc(kCoprime30 & ((1u << (t >= 30u ? 0u : t)) - 1u));
+#pragma unroll
+    for (uint32_t k = 0; k < 3; k++) e[m][k] = ctabq[row * 8u + ((s0 + k) & 7u)];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < kN; m++) {
+    if (!p[m]) continue;
+    const uint32_t a1 = a[m] | 0x10000u, nR = ~R[m];
+#pragma unroll
+    for (uint32_t k = 0; k < 3; k++) {
+      const uint32_t i = class_offset(e[m][k].x, a1, nR, p[m]);
+      if (i < S) lds_or(lds_word(i), 1u << (((i & 3u) << 3) + e[m][k].y));
+    }
+  }
+}
+
 // Strike bit `v & 31` of word `v >> 5`: v = byte * 8 + bit within the segment.
 __device__ __forceinline__ void strike_bit(uint32_t v)
 {
@@ -415,6 +465,26 @@ __device__ __forceinline__ void init_ctab(uint32_t* ctab)
   }
 }
 
+// The same entries in cofactor order: ctabq[r * 8 + s] = (the entry of the
+// class whose cofactor residue is the s-th coprime residue, that class) for
+// the r-th coprime residue b = p mod 30.  Consecutive multiples of p with
+// cofactors coprime to 30 are consecutive slots of its row (cyclically).
+__device__ __forceinline__ void init_ctabq(uint2* ctabq)
+{
+  for (uint32_t t = threadIdx.x; t < 64; t += blockDim.x) {
+    const uint32_t b = kCoprimes[t >> 3], q = kCoprimes[t & 7u];
+    uint2 e = make_uint2(0u, 0u);
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t o = kOffsets[j];
+      if ((b * q) % 30u == o % 30u) {
+        const int ej = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
+        e = make_uint2(q | ((uint32_t) (ej + 1) << 16), j);
+      }
+    }
+    ctabq[t] = e;
+  }
+}
+
 // Window index (B + 16 tid) mod P of each tile for the 16-byte run of thread
 // tid of the segment at B.  B is the same for the whole workgroup, so its
 // remainders are taken once, in scalar registers.
@@ -489,11 +559,13 @@ constexpr uint32_t kT3 = 4;
 
 // LDS after the segment: the class table, the workgroup's counters (the six
 // sums, segment and run numbers, the queue counter, the list mode's wave
-// totals), then the tier-0 primes and their magics (read once per workgroup).
+// totals), the tier-0 primes and their magics (read once per workgroup),
+// then the cofactor-ordered class table (64 uint2).
 constexpr uint32_t kCtabWords = 240;
 constexpr uint32_t kVarWords = 32;
 constexpr uint32_t kTier0Slots = PSG_TIER0_SLOTS;
-static_assert((kCtabWords + kVarWords + 3 * kTier0Slots) * 4 == PSG_SIEVE_LDS_TAIL,
+constexpr uint32_t kCtabqWords = 128;
+static_assert((kCtabWords + kVarWords + 3 * kTier0Slots + kCtabqWords) * 4 == PSG_SIEVE_LDS_TAIL,
               "LDS tail layout");
 
 // First prime index of lane `lane` in work item `it` of a segment's queue
@@ -610,6 +682,7 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
                                                uint32_t seg, uint64_t B, uint32_t* sieve,
                                                const uint32_t* ctab, uint32_t* wsum)
 {
+  const uint2* ctabq = (const uint2*) (ctab + kCtabWords + kVarWords + 3 * kTier0Slots);
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
   const uint32_t lane = tid & 63u;
@@ -793,14 +866,19 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
             if (p2) strike_tier2(p2, a2, R2, B, S, row2, pMin2, nearSmall);
           } else {
             uint32_t pa[kT3], Ra[kT3];
+            uint64_t Ma[kT3];
             pa[0] = p;
             Ra[0] = R;
+            Ma[0] = M;
 #pragma unroll
             for (uint32_t m = 0; m < kT3 - 1; m++) {
               pa[m + 1] = pX[m];
               Ra[m + 1] = pX[m] ? mod_magic(B, pX[m], MX[m]) : 0u;
+              Ma[m + 1] = MX[m];
             }
+            // lane 0 holds the item's smallest prime
             if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctab);
+            else if (small && 8ull * uniform(p) > 30ull * S) strike_three_lane<kT3>(pa, Ra, Ma, S, ctabq);
             else if (small) strike_big_lane<false, true, kT3>(pa, Ra, B, S, ctab);
             else strike_big_lane<false, false, kT3>(pa, Ra, B, S, ctab);
           }
@@ -958,8 +1036,6 @@ __device__ __forceinline__ uint32_t last_le(uint32_t n, uint32_t x, Key key)
 // mapped to their bit index: 3 bits per residue, r = 0..19 and r = 20..29.
 constexpr uint64_t kBitLo = (1ull << 12) | (2ull << 18) | (3ull << 30) | (4ull << 36) | (5ull << 48);
 constexpr uint64_t kBitHi = (6ull << 6) | (7ull << 12);
-// bit q set iff gcd(q, 30) = 1, for q = 0..29
-constexpr uint32_t kCoprime30 = 0x208A2882u;
 
 __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
 {
@@ -1133,6 +1209,7 @@ psg_sieve_count(uint64_t argsAddr)
 
   const auto* runs = gptr<PsgRun>(args.runs);
   init_ctab(ctab);
+  init_ctabq((uint2*) (ctab + kCtabWords + kVarWords + 3 * kTier0Slots));
   {
     uint32_t* t0p = ctab + kCtabWords + kVarWords;
     uint64_t* t0m = (uint64_t*) (t0p + kTier0Slots);
