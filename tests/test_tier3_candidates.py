@@ -1,0 +1,115 @@
+"""CPU check of the rule the segment kernel's tier 3 uses for primes with
+8p > 30S (strike_cands_lane in psg_sieve.hip): in a segment of S bytes at
+global byte B >= p, the crossings of p are among three classes, namely the
+first three coprime cofactors, in cofactor order, from t mod 30, where
+t = ceil((30 (B mod p) + 7) / p).
+
+The rule is restated here from its derivation (wheel-30 layout: bit j of
+byte g is 30 g + {7, 11, 13, 17, 19, 23, 29, 31}[j], the reference layout of
+src/gpu/GpuWheel.hpp:125) and compared with brute force: every multiple of p
+in the segment's value range whose cofactor is coprime to 30.  A fourth
+candidate is needed just below the 30S/8 bound, which the last test shows,
+so the bound is tight."""
+import math
+import random
+
+OFFSETS = [7, 11, 13, 17, 19, 23, 29, 31]
+COPRIMES = [1, 7, 11, 13, 17, 19, 23, 29]
+COPRIME_BITS = sum(1 << q for q in COPRIMES)
+
+
+def _class_rows():
+    """Per residue b = p mod 30: the 8 classes in cofactor order, each as
+    (cofactor residue q, byte correction e, class j), with p*q = 30*(a*q + e) + o_j."""
+    rows = {}
+    for b in COPRIMES:
+        row = []
+        for q in COPRIMES:
+            j = next(j for j, o in enumerate(OFFSETS) if (b * q) % 30 == o % 30)
+            row.append((q, (b * q - OFFSETS[j]) // 30, j))
+        rows[b] = row
+    return rows
+
+
+ROWS = _class_rows()
+
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    return all(n % d for d in range(2, math.isqrt(n) + 1))
+
+
+def crossings_brute(p, B, S):
+    """(byte within the segment, class) of every multiple of p with a cofactor
+    coprime to 30 in the segment's values [30B + 7, 30(B + S) + 7), p itself excluded."""
+    hits = set()
+    v = -(-(30 * B + 7) // p) * p
+    while v < 30 * (B + S) + 7:
+        if math.gcd(v // p, 30) == 1 and v != p:
+            g = (v - 7) // 30
+            hits.add((g - B, OFFSETS.index(v - 30 * g)))
+        v += p
+    return hits
+
+
+def crossings_candidates(p, B, S, k=3):
+    """The kernel's rule: k classes from the first coprime cofactor >= t mod 30."""
+    a, b = divmod(p, 30)
+    R = B % p
+    t = (30 * R + 7 + p - 1) // p
+    assert 1 <= t <= 30
+    tm = 0 if t == 30 else t
+    s0 = bin(COPRIME_BITS & ((1 << tm) - 1)).count("1")
+    hits = set()
+    for i in range(k):
+        q, e, j = ROWS[b][(s0 + i) % 8]
+        off = (a * q + e - R) % p
+        if off < S:
+            hits.add((off, j))
+    return hits
+
+
+def _random_prime(rng, lo, hi):
+    while True:
+        p = rng.randrange(lo, hi + 1)
+        if _is_prime(p):
+            return p
+
+
+def test_class_rows_cover_every_class():
+    for b, row in ROWS.items():
+        assert sorted(j for _, _, j in row) == list(range(8))
+        assert [q for q, _, _ in row] == COPRIMES
+
+
+def test_three_candidates_hold_every_crossing():
+    rng = random.Random(7)
+    for S in (256, 4096, 131072):
+        lo, hi = 30 * S // 8 + 1, 7 * S
+        cases = [(_random_prime(rng, lo, hi), None) for _ in range(150 if S < 131072 else 40)]
+        cases += [(p, None) for p in [x for x in range(lo, lo + 400) if _is_prime(x)][:6]]
+        for p, _ in cases:
+            for _ in range(6):
+                B = rng.randrange(p, 10 ** 15)
+                assert crossings_candidates(p, B, S) == crossings_brute(p, B, S), (p, B, S)
+
+
+def test_bound_is_tight():
+    """Below 8p > 30S a segment can hold four crossings of p, so three
+    candidates would miss one: the kernel keeps those primes on the
+    eight-class path."""
+    rng = random.Random(11)
+    S = 256
+    found = False
+    for p in [x for x in range(30 * S // 12 + 1, 30 * S // 8 + 1) if _is_prime(x)]:
+        for _ in range(200):
+            B = rng.randrange(p, 10 ** 9)
+            if len(crossings_brute(p, B, S)) == 4:
+                assert crossings_candidates(p, B, S, k=3) != crossings_brute(p, B, S)
+                assert crossings_candidates(p, B, S, k=4) == crossings_brute(p, B, S)
+                found = True
+                break
+        if found:
+            break
+    assert found
