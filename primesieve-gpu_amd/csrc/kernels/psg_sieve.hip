@@ -348,22 +348,23 @@ __device__ __forceinline__ void strike_big_lane(const uint32_t (&p)[kN], const u
     if (p[m]) strike_big<kNear, kSmall>(p[m], a[m], R[m], B, S, row[m]);
 }
 
-// The hits of the kN primes of a lane (p[m] = 0: none) with 8p > 30S and
-// p < 30 * 2^16 in a segment at B >= p (R = B mod p, M the division magic).
-// The segment's 30S values hold at most 8 consecutive multiples of such a
-// prime, so at most 3 whose cofactor is coprime to 30.  With B = Qp + R, the
-// first multiple at or after the segment's first value 30B + 7 is
+// The hits of the kN primes of a lane (p[m] = 0: none) with p < 30 * 2^16
+// in a segment at B >= p (R = B mod p, M the division magic), for primes
+// with 8p > 30S (kK = 3) or 6p > 30S (kK = 2).  The segment's 30S values
+// hold at most 8 (6) consecutive multiples of such a prime, and any 8 (6)
+// consecutive integers hold at most 3 (2) coprime to 30.  With B = Qp + R,
+// the first multiple at or after the segment's first value 30B + 7 is
 // p * (30Q + t), t = ceil((30R + 7) / p) in [1, 30], so its hits are among
-// the three classes of its cofactor-ordered row from the first coprime
-// residue >= t mod 30; each is tested exactly, as in strike_big.  Three
-// table reads and three strikes per prime instead of two row reads and
-// eight strikes, and no class offsets for the other five classes.
-template <uint32_t kN>
-__device__ __forceinline__ void strike_three_lane(const uint32_t (&p)[kN], const uint32_t (&R)[kN],
+// the kK classes of its cofactor-ordered row from the first coprime residue
+// >= t mod 30; each is tested exactly, as in strike_big.  kK table reads and
+// kK strikes per prime instead of two row reads and eight strikes, and no
+// class offsets for the other classes.
+template <uint32_t kN, uint32_t kK>
+__device__ __forceinline__ void strike_cands_lane(const uint32_t (&p)[kN], const uint32_t (&R)[kN],
                                                   const uint64_t (&M)[kN], uint32_t S, const uint2* ctabq)
 {
   uint32_t a[kN];
-  uint2 e[kN][3];
+  uint2 e[kN][kK];
 #pragma unroll
   for (uint32_t m = 0; m < kN; m++) {
     a[m] = p[m] / 30u;
@@ -372,14 +373,14 @@ __device__ __forceinline__ void strike_three_lane(const uint32_t (&p)[kN], const
     const uint32_t t = div_magic32(30u * R[m] + 6u + p[m], p[m], M[m]);  // ceil((30R + 7) / p)
     const uint32_t s0 = __popc(kCoprime30 & ((1u << (t >= 30u ? 0u : t)) - 1u));
 #pragma unroll
-    for (uint32_t k = 0; k < 3; k++) e[m][k] = ctabq[row * 8u + ((s0 + k) & 7u)];
+    for (uint32_t k = 0; k < kK; k++) e[m][k] = ctabq[row * 8u + ((s0 + k) & 7u)];
   }
 #pragma unroll
   for (uint32_t m = 0; m < kN; m++) {
     if (!p[m]) continue;
     const uint32_t a1 = a[m] | 0x10000u, nR = ~R[m];
 #pragma unroll
-    for (uint32_t k = 0; k < 3; k++) {
+    for (uint32_t k = 0; k < kK; k++) {
       const uint32_t i = class_offset(e[m][k].x, a1, nR, p[m]);
       if (i < S) lds_or(lds_word(i), 1u << (((i & 3u) << 3) + e[m][k].y));
     }
@@ -878,7 +879,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
             }
             // lane 0 holds the item's smallest prime
             if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctab);
-            else if (small && 8ull * uniform(p) > 30ull * S) strike_three_lane<kT3>(pa, Ra, Ma, S, ctabq);
+            else if (small && 6ull * uniform(p) > 30ull * S) strike_cands_lane<kT3, 2>(pa, Ra, Ma, S, ctabq);
+            else if (small && 8ull * uniform(p) > 30ull * S) strike_cands_lane<kT3, 3>(pa, Ra, Ma, S, ctabq);
             else if (small) strike_big_lane<false, true, kT3>(pa, Ra, B, S, ctab);
             else strike_big_lane<false, false, kT3>(pa, Ra, B, S, ctab);
           }
