@@ -2,7 +2,7 @@
 8p > 30S (strike_cands_lane in psg_sieve.hip): in a segment of S bytes at
 global byte B >= p, the crossings of p are among three classes, namely the
 first three coprime cofactors, in cofactor order, from t mod 30, where
-t = ceil((30 (B mod p) + 7) / p).
+t = ceil((30 (B mod p) + 7) / p); two classes when 6p > 30S.
 
 The rule is restated here from its derivation (wheel-30 layout: bit j of
 byte g is 30 g + {7, 11, 13, 17, 19, 23, 29, 31}[j], the reference layout of
@@ -93,6 +93,20 @@ def test_three_candidates_hold_every_crossing():
             for _ in range(6):
                 B = rng.randrange(p, 10 ** 15)
                 assert crossings_candidates(p, B, S) == crossings_brute(p, B, S), (p, B, S)
+
+
+def test_two_candidates_above_five_segments():
+    """6p > 30S: at most 6 consecutive multiples in a segment, at most 2 of them
+    with a cofactor coprime to 30 (the kernel's two-candidate band)."""
+    rng = random.Random(9)
+    for S in (256, 4096, 131072):
+        lo, hi = 30 * S // 6 + 1, 24 * S
+        primes = [_random_prime(rng, lo, hi) for _ in range(100 if S < 131072 else 30)]
+        primes += [x for x in range(lo, lo + 400) if _is_prime(x)][:6]
+        for p in primes:
+            for _ in range(6):
+                B = rng.randrange(p, 10 ** 15)
+                assert crossings_candidates(p, B, S, k=2) == crossings_brute(p, B, S), (p, B, S)
 
 
 def test_bound_is_tight():
