@@ -813,28 +813,40 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
       }
       // the bucket entries prefetched before tier 0 (arrived by now), then
       // the first batch in flight; a lane loads min(index, last) so that its
-      // loads stay inside the list
+      // loads stay inside the list.  A batch that lies inside the list for
+      // every lane of the wave (all but a list's last ones) is loaded and
+      // struck without the per-entry tests.
       const auto* __restrict__ bent = gptr<uint32_t>(args.buckets);
+      const auto strike_batch = [&](const uint32_t (&v)[8]) {
+        if (__all(bE + 7u * nthreads < bE1)) {
 #pragma unroll
-      for (uint32_t u = 0; u < kBucketPre; u++)
-        if (bE + u * nthreads < bE1) strike_bit(bpre[u]);
+          for (uint32_t u = 0; u < 8; u++) strike_bit(v[u]);
+        } else {
+#pragma unroll
+          for (uint32_t u = 0; u < 8; u++)
+            if (bE + u * nthreads < bE1) strike_bit(v[u]);
+        }
+      };
+      const auto load_batch = [&](uint32_t (&v)[8]) {
+        if (__all(bE + 7u * nthreads < bE1)) {
+#pragma unroll
+          for (uint32_t u = 0; u < 8; u++) v[u] = bent[bE + u * nthreads];
+        } else {
+#pragma unroll
+          for (uint32_t u = 0; u < 8; u++) v[u] = bent[min(bE + u * nthreads, bE1 - 1u)];
+        }
+      };
+      static_assert(kBucketPre == 8, "the prefetched batch is struck as one batch");
+      strike_batch(bpre);
       bE += kBucketPre * nthreads;
       uint32_t bq[8];
-      if (bE < bE1) {
-#pragma unroll
-        for (uint32_t u = 0; u < 8; u++) bq[u] = bent[min(bE + u * nthreads, bE1 - 1u)];
-      }
+      if (bE < bE1) load_batch(bq);
       while (cur < u3) {
         const uint32_t after = ticket();                 // the item after nxt, in flight
         if (bE < bE1) {
-#pragma unroll
-          for (uint32_t u = 0; u < 8; u++)
-            if (bE + u * nthreads < bE1) strike_bit(bq[u]);
+          strike_batch(bq);
           bE += 8u * nthreads;
-          if (bE < bE1) {
-#pragma unroll
-            for (uint32_t u = 0; u < 8; u++) bq[u] = bent[min(bE + u * nthreads, bE1 - 1u)];
-          }
+          if (bE < bE1) load_batch(bq);
         }
         const uint32_t kN = primeOf(nxt);
         const uint32_t pN = (kN != 0xffffffffu) ? primes[kN] : 0u;
@@ -896,9 +908,7 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
         }
       }
       if (bE < bE1) {
-#pragma unroll
-        for (uint32_t u = 0; u < 8; u++)
-          if (bE + u * nthreads < bE1) strike_bit(bq[u]);
+        strike_batch(bq);
         bE += 8u * nthreads;
       }
     }
