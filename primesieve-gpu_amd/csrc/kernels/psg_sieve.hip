@@ -117,6 +117,16 @@ __device__ __forceinline__ double to_double(uint64_t B)
   return fma((double) (uint32_t) (B >> 32), 0x1p32, (double) (uint32_t) B);   // one rounding
 }
 
+// A wave-uniform double moved into scalar registers (VALU instructions take
+// a scalar pair as an operand), so that it occupies no vector registers.
+__device__ __forceinline__ double uniform_f64(double x)
+{
+  const uint64_t u = (uint64_t) __double_as_longlong(x);
+  const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) u);
+  const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) (u >> 32));
+  return __longlong_as_double((long long) (((uint64_t) hi << 32) | lo));
+}
+
 __device__ __forceinline__ double recip(uint32_t p)
 {
   const double pd = (double) p;
@@ -1073,7 +1083,7 @@ struct BucketWalk {
   uint32_t k;        // index of the last prefetched prime
   uint32_t pn[kPrefetch];   // the next primes, in order (0: none left)
   uint32_t p;        // prime being walked (0: none)
-  uint32_t R;        // window base mod p
+  uint32_t R;        // dense: window base mod p
   uint32_t j;        // dense: current class; 8 marks a sparse prime
   uint32_t pos;      // dense: next byte of class j, relative to the window
   uint32_t qm;       // sparse: cofactor of the next multiple, mod 30
@@ -1081,20 +1091,33 @@ struct BucketWalk {
   uint32_t self;     // sparse: offset of p itself when it lies in the window, else ~0
 };
 
+// A window's first byte and first value as doubles (off by at most 2^6 and
+// 2^11), in scalar registers.
+struct WindowBase {
+  double baseF;
+  double v0F;
+};
+
+__device__ __forceinline__ WindowBase window_base(const PsgWindow& w)
+{
+  return {uniform_f64(to_double(w.base)), uniform_f64(to_double(30ull * w.base + 7u))};
+}
+
 // emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ..., with byte
 // and bit within the window (so the window segment is the hit >> (segShift +
 // 3)); returns the number of hits of this stage.
 template <typename Emit>
 __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, const PsgWindow& w,
-                                                      BucketWalk& s, uint32_t stride,
-                                                      const uint32_t* ctab, uint32_t limit,
-                                                      Emit emit)
+                                                      const WindowBase& wb, BucketWalk& s,
+                                                      uint32_t stride, const uint32_t* ctab,
+                                                      uint32_t limit, Emit emit)
 {
   const auto* __restrict__ primes = gptr<uint32_t>(a.primes);
   const uint32_t bytes = w.segs << a.segShift;
-  const double baseF = to_double(w.base);
+  const double baseF = wb.baseF;
   const uint32_t V = 30u * bytes;                 // values covered by the window (< 2^31)
-  const uint64_t v0 = 30ull * w.base + 7u;        // its first value
+  const uint64_t v0 = 30ull * w.base + 7u;        // its first value (< 2^64 - 8)
+  const double v0F = wb.v0F;
   uint32_t n = 0;
   while (n < limit) {
     // The lanes of a wave take their next primes together, once every active
@@ -1107,28 +1130,37 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
       // take the first prefetched prime and fetch the one kPrefetch ahead
       const uint32_t p = s.pn[0];
       const double rp = recip(p);
-      const uint32_t R = mod_big(w.base, baseF, p, rp);    // bucket primes are > 2^16
 #pragma unroll
       for (uint32_t i = 0; i + 1 < kPrefetch; i++) s.pn[i] = s.pn[i + 1];
       s.k += stride;
       s.pn[kPrefetch - 1] = (s.k < w.bigEnd) ? primes[s.k] : 0u;
       s.p = p;
-      s.R = R;
       if ((uint64_t) p * 8u > V) {
-        // v0 = 30*Q*p + x with x = 30R + 7 < 31p; the first multiple of p at
-        // or after v0 is v0 + off with cofactor 30*Q + ceil(x / p)
-        const uint64_t x = 30ull * R + 7u;
-        uint32_t t = (uint32_t) floor((double) x * rp);       // within one of x / p
-        int64_t u = (int64_t) (x - (uint64_t) t * p);
-        if (u < 0) { t--; u += p; }
-        if (u >= (int64_t) p) { t++; u -= p; }
-        s.off = u ? p - (uint32_t) u : 0u;
-        s.qm = (t + (u ? 1u : 0u)) % 30u;
+        // The first multiple of p at or after v0, from one quotient:
+        // q = floor(v0F * rp) is within one of v0 / p (v0 / p < 2^48, and the
+        // product's relative error is below 2^-50), r = v0 - q p exactly in
+        // 64-bit integers, corrected into [0, p); the multiple is m p with
+        // cofactor m = q + (r != 0), at offset m p - v0 = p - r (or 0).
+        const double qf = floor(v0F * rp);
+        const double qh = floor(qf * 0x1p-32);
+        const uint32_t qlo = (uint32_t) (qf - qh * 0x1p32);   // exact: qf < 2^53
+        const uint32_t qhi = (uint32_t) qh;
+        int64_t r = (int64_t) (v0 - (uint64_t) qlo * p - ((uint64_t) (qhi * p) << 32));
+        double m = qf;
+        if (r < 0) { r += p; m -= 1.0; }
+        if (r >= (int64_t) p) { r -= p; m += 1.0; }
+        if (r) m += 1.0;
+        s.off = r ? p - (uint32_t) r : 0u;
+        // m mod 30: m * (1/30) is within 2^-7 of m / 30 (m < 2^49), so the
+        // floor is exact except that a multiple of 30 may give 30 instead of 0
+        const uint32_t qm = (uint32_t) fma(floor(m * (1.0 / 30.0)), -30.0, m);
+        s.qm = (qm == 30u) ? 0u : qm;
         s.self = (p >= v0 && p - v0 < V) ? (uint32_t) (p - v0) : 0xffffffffu;
         s.j = 8u;
       } else {
+        s.R = mod_big(w.base, baseF, p, rp);                 // bucket primes are > 2^16
         s.j = 0u;
-        s.pos = class_start(p, R, w.base, 0u, ctab);
+        s.pos = class_start(p, s.R, w.base, 0u, ctab);
       }
     }
     if (s.p == 0u) continue;                  // idle until the wave's next primes
@@ -1367,9 +1399,10 @@ psg_bucket_fill(uint64_t argsAddr)
     }
     st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
     const uint32_t stride = nthreads * w.nChunks;
+    const WindowBase wb = window_base(w);
     __syncthreads();
     for (;;) {
-      const uint32_t n = bucket_walk_stage(a, w, st, stride, ctab, kStageF,
+      const uint32_t n = bucket_walk_stage(a, w, wb, st, stride, ctab, kStageF,
           [&](uint32_t i, uint32_t seg, uint32_t hit) {
             stage[i * nthreads + tid] = hit;
             atomicAdd(&hist[(seg << cs) | copy], 1u);
