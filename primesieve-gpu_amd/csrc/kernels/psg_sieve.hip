@@ -1064,24 +1064,21 @@ __device__ __forceinline__ uint32_t bit_of_residue(uint32_t r)
   return (uint32_t) ((r < 20u ? (kBitLo >> (3u * r)) : (kBitHi >> (3u * (r - 20u)))) & 7u);
 }
 
-// Primes a thread has in flight ahead of the one it walks: a visit is often
-// shorter than the latency of a prime's load.
-constexpr uint32_t kPrefetch = 4;
+// Primes a thread has loaded ahead of the one it walks: the current group
+// (resident; taken in order, shifted down at every take) and the next group
+// of kGroup (in flight).  The next group's loads are issued when the current
+// group is refilled from it, so they have kGroup takes to arrive.  The refill
+// copies them with DPP moves: a plain copy would be deferred to the end of the
+// refill block, after the next loads, which would then land in temporaries
+// copied into place at once -- a wait on the loads just issued at every refill
+// (and, with a rotating queue of single primes, at every take).
+constexpr uint32_t kGroup = 4;
 
-// Resumable walk of one thread over its share of a window's big primes.  The
-// big primes are cut into tiles of blockDim consecutive primes and the tiles
-// are dealt round-robin to the window's chunks (tile m*nChunks + c), so that
-// every wave loads consecutive primes (coalesced) and every chunk gets a
-// similar mix of small and large primes.  Per prime:
-//   dense (>= 8 multiples in the window): the 8 residue classes, each an
-//     arithmetic progression of bytes with stride p;
-//   sparse: the multiples of p in the window's value range directly, keeping
-//     those whose cofactor is coprime to 30 -- one small division instead of
-//     eight class starts.
-// A stage stops after `limit` hits; the walk resumes there in the next stage.
 struct BucketWalk {
-  uint32_t k;        // index of the last prefetched prime
-  uint32_t pn[kPrefetch];   // the next primes, in order (0: none left)
+  uint32_t k;        // index of the next group's first prime
+  uint32_t q0, q1, q2, q3;  // the current group, next prime first (0: none left)
+  uint32_t r0, r1, r2, r3;  // the next group, in flight
+  uint32_t t;        // primes of the current group taken (uniform over the wave)
   uint32_t p;        // prime being walked (0: none)
   uint32_t R;        // dense: window base mod p
   uint32_t j;        // dense: current class; 8 marks a sparse prime
@@ -1090,6 +1087,70 @@ struct BucketWalk {
   uint32_t off;      // sparse: next multiple minus the window's first value (< 2^31)
   uint32_t self;     // sparse: offset of p itself when it lies in the window, else ~0
 };
+
+// The thread's next prime (0: none left), not yet taken.
+__device__ __forceinline__ uint32_t walk_peek(const BucketWalk& s)
+{
+  return s.q0;
+}
+
+// Load a group: primes k + i * stride, the index clamped to end - 1 (end >= 1;
+// unpredicated loads, so that they can write their registers directly; the
+// primes past end are zeroed when the group moves in).
+__device__ __forceinline__ void walk_load(uint32_t& g0, uint32_t& g1, uint32_t& g2, uint32_t& g3,
+                                          const uint32_t* primes, uint32_t k, uint32_t stride,
+                                          uint32_t end)
+{
+  const auto ld = [&](uint32_t i) { return primes[min(k + i * stride, end - 1u)]; };
+  g0 = ld(0);
+  g1 = ld(1);
+  g2 = ld(2);
+  g3 = ld(3);
+}
+
+// A register copy the compiler keeps where it is written (a DPP move with the
+// identity lane pattern).
+__device__ __forceinline__ uint32_t copy_now(uint32_t x)
+{
+  return (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0xE4, 0xF, 0xF, false);
+}
+
+// The thread's primes are first, first + stride, first + 2 stride, ...
+__device__ __forceinline__ void walk_init(BucketWalk& s, const uint32_t* primes, uint32_t first,
+                                          uint32_t stride, uint32_t end)
+{
+  walk_load(s.q0, s.q1, s.q2, s.q3, primes, first, stride, end);
+  s.q0 = (first < end) ? s.q0 : 0u;
+  s.q1 = (first + stride < end) ? s.q1 : 0u;
+  s.q2 = (first + 2u * stride < end) ? s.q2 : 0u;
+  s.q3 = (first + 3u * stride < end) ? s.q3 : 0u;
+  s.k = first + kGroup * stride;
+  walk_load(s.r0, s.r1, s.r2, s.r3, primes, s.k, stride, end);
+  s.t = 0u;
+}
+
+// Step past the next prime (all lanes of a wave together, so t stays
+// uniform); when the current group is used up the next one moves in and the
+// group after it is loaded.
+__device__ __forceinline__ void walk_advance(BucketWalk& s, const uint32_t* primes, uint32_t stride,
+                                             uint32_t end)
+{
+  if (++s.t == kGroup) {
+    s.t = 0u;
+    s.q0 = (s.k < end) ? copy_now(s.r0) : 0u;
+    s.q1 = (s.k + stride < end) ? copy_now(s.r1) : 0u;
+    s.q2 = (s.k + 2u * stride < end) ? copy_now(s.r2) : 0u;
+    s.q3 = (s.k + 3u * stride < end) ? copy_now(s.r3) : 0u;
+    // nothing of the next loads (their zeroing included) moves above the copies
+    __builtin_amdgcn_sched_barrier(0);
+    s.k += kGroup * stride;
+    walk_load(s.r0, s.r1, s.r2, s.r3, primes, s.k, stride, end);
+  } else {
+    s.q0 = s.q1;
+    s.q1 = s.q2;
+    s.q2 = s.q3;
+  }
+}
 
 // A window's first byte and first value as doubles (off by at most 2^6 and
 // 2^11), in scalar registers.
@@ -1125,15 +1186,11 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
     // multiple or class start) then runs once per prime for the whole wave
     // instead of interleaving with the hit loops of other lanes.
     if (__all(s.p == 0u)) {
-      if (__all(s.pn[0] == 0u)) break;
-      if (s.pn[0] == 0u) continue;            // this lane is done; others take primes
-      // take the first prefetched prime and fetch the one kPrefetch ahead
-      const uint32_t p = s.pn[0];
+      const uint32_t p = walk_peek(s);
+      if (__all(p == 0u)) break;
+      walk_advance(s, primes, stride, w.bigEnd);
+      if (p == 0u) continue;                  // this lane is done; others take primes
       const double rp = recip(p);
-#pragma unroll
-      for (uint32_t i = 0; i + 1 < kPrefetch; i++) s.pn[i] = s.pn[i + 1];
-      s.k += stride;
-      s.pn[kPrefetch - 1] = (s.k < w.bigEnd) ? primes[s.k] : 0u;
       s.p = p;
       if ((uint64_t) p * 8u > V) {
         // The first multiple of p at or after v0, from one quotient:
@@ -1391,14 +1448,9 @@ psg_bucket_fill(uint64_t argsAddr)
     for (uint32_t i = tid; i < HW; i += nthreads) hist[i] = 0;
 
     BucketWalk st;
-    st.k = a.bigBegin + c * nthreads + tid;
-#pragma unroll
-    for (uint32_t i = 0; i < kPrefetch; i++) {
-      if (i) st.k += nthreads * w.nChunks;
-      st.pn[i] = (st.k < w.bigEnd) ? gptr<uint32_t>(a.primes)[st.k] : 0u;
-    }
-    st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
     const uint32_t stride = nthreads * w.nChunks;
+    walk_init(st, gptr<uint32_t>(a.primes), a.bigBegin + c * nthreads + tid, stride, w.bigEnd);
+    st.p = 0u; st.R = 0u; st.j = 0u; st.pos = 0u; st.qm = 0u; st.off = 0u; st.self = 0u;
     const WindowBase wb = window_base(w);
     __syncthreads();
     for (;;) {
@@ -1407,7 +1459,7 @@ psg_bucket_fill(uint64_t argsAddr)
             stage[i * nthreads + tid] = hit;
             atomicAdd(&hist[(seg << cs) | copy], 1u);
           });
-      const int more = __syncthreads_or((st.p != 0u || st.pn[0] != 0u) ? 1 : 0);
+      const int more = __syncthreads_or((st.p != 0u || walk_peek(st) != 0u) ? 1 : 0);
 
       // exclusive prefix of the counts; thread i owns segment i and its copies
       uint32_t cc[4] = {0u, 0u, 0u, 0u};
