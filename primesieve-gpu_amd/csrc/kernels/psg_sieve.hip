@@ -1496,9 +1496,22 @@ psg_bucket_fill(uint64_t argsAddr)
           *(uint2*) &hist[tid << 1] = make_uint2(f, f + cc[0]);
         if (cnt) at = atomicAdd(&listCount[tid], cnt);   // consumed after the scatter
       }
+      // The stage passes below cover only the slots in use: a wave reads
+      // back, and scatters, the groups of eight in which any of its threads
+      // staged a hit, and the write-out stops at the stage's total.  The last
+      // stage of an item is often far from full, and on the batch of short
+      // intervals such stages are most of the work.
       uint32_t e[kStageF];
 #pragma unroll
-      for (uint32_t u = 0; u < kStageF; u++) e[u] = stage[u * nthreads + tid];
+      for (uint32_t g = 0; g < kStageF; g += 8) {
+        if (__any(g < n)) {
+#pragma unroll
+          for (uint32_t q = 0; q < 8; q++) e[g + q] = stage[(g + q) * nthreads + tid];
+        } else {
+#pragma unroll
+          for (uint32_t q = 0; q < 8; q++) e[g + q] = 0u;
+        }
+      }
       __syncthreads();                                   // cursors set, stage read
 
       // Scatter by segment, eight hits at a time: their cursor atomics first,
@@ -1508,6 +1521,7 @@ psg_bucket_fill(uint64_t argsAddr)
       const auto scatter = [&](auto full) {
 #pragma unroll
         for (uint32_t g = 0; g < kStageF; g += 8) {
+          if (!full.value && !__any(g < n)) break;
           uint32_t slot[8];
 #pragma unroll
           for (uint32_t q = 0; q < 8; q++)
@@ -1533,6 +1547,7 @@ psg_bucket_fill(uint64_t argsAddr)
       // and they store nothing.
 #pragma unroll
       for (uint32_t g = 0; g < kStageF; g += 8) {
+        if (g * nthreads >= total) break;                // total is uniform
         uint32_t v[8];
         uint64_t pl[8];
 #pragma unroll
