@@ -81,21 +81,42 @@ __device__ __constant__ uint32_t kOffsets[8] = {7, 11, 13, 17, 19, 23, 29, 31};
 // the residues coprime to 30, ascending, and as a bit set (bit q for q = 0..29)
 __device__ __constant__ uint32_t kCoprimes[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 constexpr uint32_t kCoprime30 = 0x208A2882u;
+// slots per row of the cofactor-ordered class table (8 coprime residues, repeated)
+constexpr uint32_t kCtabqSlots = 16;
 
 // R = B mod p for B < 2^63 using M = ceil(2^(64+L)/p):
 // floor(B/p) = mulhi(B, M) >> L.  Only the low 32 bits of B - q*p matter.
+__device__ __forceinline__ uint32_t magic_shift(uint32_t p)
+{
+  return 31u - (uint32_t) __clz((int) (p | 1u));
+}
+
 __device__ __forceinline__ uint32_t mod_magic(uint64_t B, uint32_t p, uint64_t M)
 {
-  const uint32_t L = 31u - (uint32_t) __clz((int) p);
-  const uint64_t q = __umul64hi(B, M) >> L;
+  const uint64_t q = __umul64hi(B, M) >> magic_shift(p);
   return (uint32_t) B - (uint32_t) q * p;
+}
+
+// The same for p < 2^31 with three of the four partial products of B * M:
+// H = B_hi M_hi + hi(B_hi M_lo) + hi(B_lo M_hi) drops B_lo M_lo / 2^64 and
+// two fractional parts, each below 1, so it is at most 2 below the exact
+// high word.  With L >= 1 the quotient H >> L is then floor(B / p) or one
+// less, and B - q p lies in [0, 2p) (below 2^32): one subtraction of p fixes
+// it (min(r, r - p): r - p wraps above r when r < p).
+__device__ __forceinline__ uint32_t mod_magic31(uint64_t B, uint32_t p, uint64_t M)
+{
+  const uint32_t Bl = (uint32_t) B, Bh = (uint32_t) (B >> 32);
+  const uint32_t Ml = (uint32_t) M, Mh = (uint32_t) (M >> 32);
+  const uint64_t H = (uint64_t) Bh * Mh + __umulhi(Bh, Ml) + __umulhi(Bl, Mh);
+  const uint32_t r = Bl - (uint32_t) (H >> magic_shift(p)) * p;
+  return min(r, r - p);
 }
 
 // floor(y / p) for y < 2^32 with the same magic: the high 64 bits of y * M
 // (y * M_hi plus the carry word of y * M_lo), shifted by L.
 __device__ __forceinline__ uint32_t div_magic32(uint32_t y, uint32_t p, uint64_t M)
 {
-  const uint32_t L = 31u - (uint32_t) __clz((int) (p | 1u));
+  const uint32_t L = magic_shift(p);
   const uint64_t hi = (uint64_t) y * (uint32_t) (M >> 32) + __umulhi(y, (uint32_t) M);
   return (uint32_t) (hi >> 32) >> L;
 }
@@ -365,25 +386,26 @@ __device__ __forceinline__ void strike_big_lane(const uint32_t (&p)[kN], const u
 // consecutive integers hold at most 3 (2) coprime to 30.  With B = Qp + R,
 // the first multiple at or after the segment's first value 30B + 7 is
 // p * (30Q + t), t = ceil((30R + 7) / p) in [1, 30], so its hits are among
-// the kK classes of its cofactor-ordered row from the first coprime residue
-// >= t mod 30; each is tested exactly, as in strike_big.  kK table reads and
-// kK strikes per prime instead of two row reads and eight strikes, and no
-// class offsets for the other classes.
+// the kK slots of its cofactor-ordered row from the first coprime residue
+// >= t (t = 30 is residue 0, the row's slot 8, which repeats slot 0); each is
+// tested exactly, as in strike_big.  kK adjacent table slots and kK strikes
+// per prime instead of two row reads and eight strikes, and no class offsets
+// for the other classes.  A slot holds the class's bit, so a strike's word
+// mask is that bit shifted by 8 * (i mod 4): (i << 3) mod 32.
 template <uint32_t kN, uint32_t kK>
 __device__ __forceinline__ void strike_cands_lane(const uint32_t (&p)[kN], const uint32_t (&R)[kN],
                                                   const uint64_t (&M)[kN], uint32_t S, const uint2* ctabq)
 {
   uint32_t a[kN];
-  uint2 e[kN][kK];
+  uint64_t e[kN][kK];         // slots read whole (8-byte LDS reads, two per instruction)
 #pragma unroll
   for (uint32_t m = 0; m < kN; m++) {
     a[m] = p[m] / 30u;
     const uint32_t b = p[m] - a[m] * 30u;
-    const uint32_t row = __popc(kCoprime30 & ((1u << b) - 1u));          // rank of b
     const uint32_t t = div_magic32(30u * R[m] + 6u + p[m], p[m], M[m]);  // ceil((30R + 7) / p)
-    const uint32_t s0 = __popc(kCoprime30 & ((1u << (t >= 30u ? 0u : t)) - 1u));
+    const uint64_t* row = (const uint64_t*) ctabq + ((b >> 1) * kCtabqSlots + __popc(kCoprime30 & ((1u << t) - 1u)));
 #pragma unroll
-    for (uint32_t k = 0; k < kK; k++) e[m][k] = ctabq[row * 8u + ((s0 + k) & 7u)];
+    for (uint32_t k = 0; k < kK; k++) e[m][k] = row[k];
   }
 #pragma unroll
   for (uint32_t m = 0; m < kN; m++) {
@@ -391,8 +413,8 @@ __device__ __forceinline__ void strike_cands_lane(const uint32_t (&p)[kN], const
     const uint32_t a1 = a[m] | 0x10000u, nR = ~R[m];
 #pragma unroll
     for (uint32_t k = 0; k < kK; k++) {
-      const uint32_t i = class_offset(e[m][k].x, a1, nR, p[m]);
-      if (i < S) lds_or(lds_word(i), 1u << (((i & 3u) << 3) + e[m][k].y));
+      const uint32_t i = class_offset((uint32_t) e[m][k], a1, nR, p[m]);
+      if (i < S) lds_or(lds_word(i), (uint32_t) (e[m][k] >> 32) << ((i << 3) & 31u));
     }
   }
 }
@@ -476,20 +498,24 @@ __device__ __forceinline__ void init_ctab(uint32_t* ctab)
   }
 }
 
-// The same entries in cofactor order: ctabq[r * 8 + s] = (the entry of the
-// class whose cofactor residue is the s-th coprime residue, that class) for
-// the r-th coprime residue b = p mod 30.  Consecutive multiples of p with
-// cofactors coprime to 30 are consecutive slots of its row (cyclically).
+// The same entries in cofactor order: ctabq[(b >> 1) * 16 + s] = (the entry
+// of the class whose cofactor residue is the (s mod 8)-th coprime residue,
+// the bit of that class) for b = p mod 30 coprime to 30 (one row per odd b;
+// the rows of odd b sharing a factor with 30 are zero).
+// Consecutive multiples of p with cofactors coprime to 30 are consecutive
+// slots of its row; the row repeats its 8 slots once (kCtabqSlots), so that
+// the slots from any s <= 8 on need no wrap.
+
 __device__ __forceinline__ void init_ctabq(uint2* ctabq)
 {
-  for (uint32_t t = threadIdx.x; t < 64; t += blockDim.x) {
-    const uint32_t b = kCoprimes[t >> 3], q = kCoprimes[t & 7u];
+  for (uint32_t t = threadIdx.x; t < 15u * kCtabqSlots; t += blockDim.x) {
+    const uint32_t b = 2u * (t / kCtabqSlots) + 1u, q = kCoprimes[t & 7u];
     uint2 e = make_uint2(0u, 0u);
-    for (uint32_t j = 0; j < 8; j++) {
+    for (uint32_t j = 0; j < 8 && ((kCoprime30 >> b) & 1u); j++) {
       const uint32_t o = kOffsets[j];
       if ((b * q) % 30u == o % 30u) {
         const int ej = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
-        e = make_uint2(q | ((uint32_t) (ej + 1) << 16), j);
+        e = make_uint2(q | ((uint32_t) (ej + 1) << 16), 1u << j);
       }
     }
     ctabq[t] = e;
@@ -571,11 +597,11 @@ constexpr uint32_t kT3 = 4;
 // LDS after the segment: the class table, the workgroup's counters (the six
 // sums, segment and run numbers, the queue counter, the list mode's wave
 // totals), the tier-0 primes and their magics (read once per workgroup),
-// then the cofactor-ordered class table (64 uint2).
+// then the cofactor-ordered class table (15 rows of 16 uint2).
 constexpr uint32_t kCtabWords = 240;
 constexpr uint32_t kVarWords = 32;
 constexpr uint32_t kTier0Slots = PSG_TIER0_SLOTS;
-constexpr uint32_t kCtabqWords = 128;
+constexpr uint32_t kCtabqWords = 2u * 15u * kCtabqSlots;
 static_assert((kCtabWords + kVarWords + 3 * kTier0Slots + kCtabqWords) * 4 == PSG_SIEVE_LDS_TAIL,
               "LDS tail layout");
 
@@ -710,13 +736,15 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
   constexpr uint32_t kBucketPre = 8;
   uint32_t bpre[kBucketPre];
   uint32_t bE = 0, bE1 = 0;
+  uint32_t bW = 0;                  // bE - tid, the same for every thread (scalar)
   if (args.buckets) {
     const auto* __restrict__ offs = gptr<uint32_t>(args.listOff);
     const auto* __restrict__ ent = gptr<uint32_t>(args.buckets);
     const uint32_t e0 = offs[seg];
     const uint32_t room = offs[seg + 1u] - e0;
     bE1 = e0 + min(gptr<uint32_t>(args.listCount)[seg], room);
-    bE = e0 + tid;
+    bW = (uint32_t) __builtin_amdgcn_readfirstlane((int) e0);
+    bE = bW + tid;
 #pragma unroll
     for (uint32_t u = 0; u < kBucketPre; u++) {
       const uint32_t idx = bE + u * nthreads;
@@ -751,7 +779,7 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
       for (uint32_t b = 0; b < t0; b += 8) {
         const uint32_t k = b + slot;                     // t0 <= kTier0Slots
         const uint32_t p = (k < t0) ? t0p[k] : 0u;
-        const uint32_t R = p ? mod_magic(Bs, p, t0m[k]) : 0u;
+        const uint32_t R = p ? mod_magic31(Bs, p, t0m[k]) : 0u;
         const uint32_t st = !p ? 0u : nearSmall ? class_start<true>(p, R, Bs, j, ctab)
                                                : class_start<false>(p, R, Bs, j, ctab);
         // all eight permutes first: one wait for the batch, not one per prime
@@ -839,8 +867,10 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
       };
       const auto load_batch = [&](uint32_t (&v)[8]) {
         if (__all(bE + 7u * nthreads < bE1)) {
+          // scalar bases plus the thread's fixed offset: no vector address
+          // arithmetic per load
 #pragma unroll
-          for (uint32_t u = 0; u < 8; u++) v[u] = bent[bE + u * nthreads];
+          for (uint32_t u = 0; u < 8; u++) v[u] = (bent + (bW + u * nthreads))[tid];
         } else {
 #pragma unroll
           for (uint32_t u = 0; u < 8; u++) v[u] = bent[min(bE + u * nthreads, bE1 - 1u)];
@@ -849,6 +879,7 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
       static_assert(kBucketPre == 8, "the prefetched batch is struck as one batch");
       strike_batch(bpre);
       bE += kBucketPre * nthreads;
+      bW += kBucketPre * nthreads;
       uint32_t bq[8];
       if (bE < bE1) load_batch(bq);
       while (cur < u3) {
@@ -856,6 +887,7 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
         if (bE < bE1) {
           strike_batch(bq);
           bE += 8u * nthreads;
+          bW += 8u * nthreads;
           if (bE < bE1) load_batch(bq);
         }
         const uint32_t kN = primeOf(nxt);
@@ -871,8 +903,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
           MNX[m] = ok ? magics[km] : 0ull;
         }
         if (p) {
-          const uint32_t R = mod_magic(B, p, M);
           if (cur < u1) {
+            const uint32_t R = mod_magic31(B, p, M);             // p <= S < 2^31
             const uint32_t i = nearSmall ? class_start<true>(p, R, B, j, ctab)
                                          : class_start<false>(p, R, B, j, ctab);
             strike_class(i, p, j, S);
@@ -880,7 +912,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
             // two primes per lane (k, k + 64), both class rows read before
             // the first strike; lane 0 holds each half's smallest prime
             const uint32_t p2 = pX[0];
-            const uint32_t R2 = p2 ? mod_magic(B, p2, MX[0]) : 0u;
+            const uint32_t R = mod_magic31(B, p, M);
+            const uint32_t R2 = p2 ? mod_magic31(B, p2, MX[0]) : 0u;
             const uint32_t pMin = uniform(p), pMin2 = uniform(p2);
             const uint32_t a = p / 30u, a2 = p2 / 30u;
             const ClassRow row = class_row(ctab, p - a * 30u);
@@ -888,16 +921,23 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
             strike_tier2(p, a, R, B, S, row, pMin, nearSmall);
             if (p2) strike_tier2(p2, a2, R2, B, S, row2, pMin2, nearSmall);
           } else {
+            // remainders (p < 30 * 2^16 < 2^31 when `small`; lanes without a
+            // prime, p = 0 and M = 0, get an unused value)
             uint32_t pa[kT3], Ra[kT3];
             uint64_t Ma[kT3];
             pa[0] = p;
-            Ra[0] = R;
             Ma[0] = M;
 #pragma unroll
             for (uint32_t m = 0; m < kT3 - 1; m++) {
               pa[m + 1] = pX[m];
-              Ra[m + 1] = pX[m] ? mod_magic(B, pX[m], MX[m]) : 0u;
               Ma[m + 1] = MX[m];
+            }
+            if (small) {
+#pragma unroll
+              for (uint32_t m = 0; m < kT3; m++) Ra[m] = mod_magic31(B, pa[m], Ma[m]);
+            } else {
+#pragma unroll
+              for (uint32_t m = 0; m < kT3; m++) Ra[m] = mod_magic(B, pa[m], Ma[m]);
             }
             // lane 0 holds the item's smallest prime
             if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctab);

@@ -1,15 +1,17 @@
 """CPU check of the rule the segment kernel's tier 3 uses for primes with
-8p > 30S (strike_cands_lane in psg_sieve.hip): in a segment of S bytes at
-global byte B >= p, the crossings of p are among three classes, namely the
-first three coprime cofactors, in cofactor order, from t mod 30, where
-t = ceil((30 (B mod p) + 7) / p); two classes when 6p > 30S.
+16p > 30S (strike_cands_lane in psg_sieve.hip): in a segment of S bytes at
+global byte B >= p, the crossings of p are among K classes, namely the first
+K coprime cofactors, in cofactor order, from t mod 30, where
+t = ceil((30 (B mod p) + 7) / p).  The segment's values hold at most m
+consecutive multiples of p when mp > 30S, and m consecutive integers hold at
+most K(m) coprime to 30: K = 2, 3, 4, 5 for m = 6, 8, 12, 16.
 
 The rule is restated here from its derivation (wheel-30 layout: bit j of
 byte g is 30 g + {7, 11, 13, 17, 19, 23, 29, 31}[j], the reference layout of
 src/gpu/GpuWheel.hpp:125) and compared with brute force: every multiple of p
-in the segment's value range whose cofactor is coprime to 30.  A fourth
-candidate is needed just below the 30S/8 bound, which the last test shows,
-so the bound is tight."""
+in the segment's value range whose cofactor is coprime to 30.  Just below
+each band one more candidate is needed, which the last test shows, so the
+bounds are tight."""
 import math
 import random
 
@@ -54,13 +56,16 @@ def crossings_brute(p, B, S):
 
 
 def crossings_candidates(p, B, S, k=3):
-    """The kernel's rule: k classes from the first coprime cofactor >= t mod 30."""
+    """The kernel's rule: k classes from the first coprime cofactor >= t mod 30.
+    Its table row holds the 8 classes twice (16 slots), and the first slot is
+    the number of coprime residues below t, 0..8 (t = 30 gives 8, which is
+    residue 0's slot repeated)."""
     a, b = divmod(p, 30)
     R = B % p
     t = (30 * R + 7 + p - 1) // p
     assert 1 <= t <= 30
-    tm = 0 if t == 30 else t
-    s0 = bin(COPRIME_BITS & ((1 << tm) - 1)).count("1")
+    s0 = bin(COPRIME_BITS & ((1 << t) - 1)).count("1")
+    assert s0 + k <= 16
     hits = set()
     for i in range(k):
         q, e, j = ROWS[b][(s0 + i) % 8]
@@ -109,6 +114,30 @@ def test_two_candidates_above_five_segments():
                 assert crossings_candidates(p, B, S, k=2) == crossings_brute(p, B, S), (p, B, S)
 
 
+def test_max_coprimes_in_consecutive_integers():
+    """K(m), the most integers coprime to 30 among m consecutive ones, for the
+    bands the kernel uses (m = 6, 8, 12, 16 -> K = 2, 3, 4, 5)."""
+    cop = [math.gcd(x, 30) == 1 for x in range(30)]
+    K = {m: max(sum(cop[(s + i) % 30] for i in range(m)) for s in range(30)) for m in range(1, 31)}
+    assert [K[m] for m in (6, 8, 12, 16)] == [2, 3, 4, 5]
+    assert [K[m] for m in (7, 9, 13, 17)] == [3, 4, 5, 6]
+
+
+def test_four_and_five_candidates():
+    """12p > 30S: four candidates; 16p > 30S: five (the kernel's bands between
+    30S/16 and 30S/8)."""
+    rng = random.Random(13)
+    for k, m in ((4, 12), (5, 16)):
+        for S in (256, 4096, 131072):
+            lo, hi = 30 * S // m + 1, 30 * S // (m - 4)
+            primes = [_random_prime(rng, lo, hi) for _ in range(100 if S < 131072 else 30)]
+            primes += [x for x in range(lo, lo + 400) if _is_prime(x)][:6]
+            for p in primes:
+                for _ in range(6):
+                    B = rng.randrange(p, 10 ** 15)
+                    assert crossings_candidates(p, B, S, k=k) == crossings_brute(p, B, S), (k, p, B, S)
+
+
 def test_bound_is_tight():
     """Below 8p > 30S a segment can hold four crossings of p, so three
     candidates would miss one: the kernel keeps those primes on the
@@ -127,3 +156,23 @@ def test_bound_is_tight():
         if found:
             break
     assert found
+
+
+def test_four_and_five_candidate_bounds_are_tight():
+    """Below 12p > 30S a segment can hold five crossings, below 16p > 30S six."""
+    rng = random.Random(17)
+    S = 256
+    for k, m in ((4, 12), (5, 16)):
+        found = False
+        for p in [x for x in range(30 * S // (m + 4) + 1, 30 * S // m + 1) if _is_prime(x)]:
+            for _ in range(400):
+                B = rng.randrange(p, 10 ** 9)
+                brute = crossings_brute(p, B, S)
+                if len(brute) == k + 1:
+                    assert crossings_candidates(p, B, S, k=k) != brute
+                    assert crossings_candidates(p, B, S, k=k + 1) == brute
+                    found = True
+                    break
+            if found:
+                break
+        assert found, k
