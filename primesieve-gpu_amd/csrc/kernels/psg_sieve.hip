@@ -605,27 +605,31 @@ constexpr uint32_t kCtabqWords = 2u * 15u * kCtabqSlots;
 static_assert((kCtabWords + kVarWords + 3 * kTier0Slots + kCtabqWords) * 4 == PSG_SIEVE_LDS_TAIL,
               "LDS tail layout");
 
-// First prime index of lane `lane` in work item `it` of a segment's queue
-// (see strike_segment), or ~0: items [0, u1) hold 8 primes of [t0, t1) each
-// (lane = prime * 8 + class), [u1, u2) 128 primes of [t1, t2) (2 per lane:
-// index, index + 64), [u2, u3) 64 * kT3 primes of [t2, t3) (lane m-th prime:
-// index + 64 m).  `it` is uniform.
-__device__ __forceinline__ uint32_t item_prime(uint32_t it, uint32_t lane, uint32_t t0, uint32_t t1,
-                                               uint32_t t2, uint32_t t3, uint32_t u1, uint32_t u2,
-                                               uint32_t u3)
+// The primes of work item `it` of a segment's queue (see strike_segment):
+// primes[kb + i] for i < n, with kb and n uniform (n = 0 past the queue).
+// Items [0, u1) hold 8 primes of [t0, t1) each (lane = prime * 8 + class:
+// slot i = lane / 8), [u1, u2) 128 primes of [t1, t2) (2 per lane: slots
+// lane, lane + 64), [u2, u3) 64 * kT3 primes of [t2, t3) (lane's m-th prime:
+// slot lane + 64 m).  `it` is uniform.
+__device__ __forceinline__ void item_span(uint32_t it, uint32_t t0, uint32_t t1, uint32_t t2,
+                                          uint32_t t3, uint32_t u1, uint32_t u2, uint32_t u3,
+                                          uint32_t& kb, uint32_t& n)
 {
-  uint32_t k, end;
+  uint32_t end;
   if (it < u1) {
-    k = t0 + (it << 3) + (lane >> 3);
+    kb = t0 + (it << 3);
     end = t1;
   } else if (it < u2) {
-    k = t1 + (it - u1) * 128u + lane;
+    kb = t1 + (it - u1) * 128u;
     end = t2;
-  } else {
-    k = t2 + (it - u2) * (64u * kT3) + lane;
+  } else if (it < u3) {
+    kb = t2 + (it - u2) * (64u * kT3);
     end = t3;
+  } else {
+    kb = 0;
+    end = 0;
   }
-  return (it < u3 && k < end) ? k : 0xffffffffu;
+  n = end - min(kb, end);
 }
 
 // List mode of step 3: write the segment's primes (ascending) instead of
@@ -830,25 +834,37 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
         const uint32_t g1 = (uint32_t) (((uint64_t) (t + 1u) * rmix) >> 32);
         return (g1 > g0) ? u2 + g0 : t - g0;
       };
-      const auto primeOf = [&](uint32_t it) { return item_prime(it, lane, t0, t1, t2, t3, u1, u2, u3); };
+      // An item's primes and magics, lane slot first (p = 0, M = 0: none).
+      // The item's first prime index is uniform, so the loads take scalar
+      // bases and the lane's fixed slot offset.
+      const auto load_item = [&](uint32_t it, uint32_t& p0, uint64_t& M0, uint32_t (&pm)[kT3 - 1],
+                                 uint64_t (&Mm)[kT3 - 1]) {
+        uint32_t kb, n;
+        item_span(it, t0, t1, t2, t3, u1, u2, u3, kb, n);
+        const auto* __restrict__ pp = primes + kb;
+        const auto* __restrict__ mp = magics + kb;
+        const uint32_t sl = (it < u1) ? (lane >> 3) : lane;
+        const uint32_t more = (it >= u2) ? kT3 - 1u : (it >= u1) ? 1u : 0u;   // further primes per lane
+        p0 = (sl < n) ? pp[sl] : 0u;
+        M0 = (sl < n) ? mp[sl] : 0ull;
+#pragma unroll
+        for (uint32_t m = 0; m < kT3 - 1; m++) {
+          const bool ok = m < more && (int) sl < (int) n - (int) (64u * (m + 1u));   // n < 2^31
+          pm[m] = ok ? pp[sl + 64u * (m + 1u)] : 0u;
+          Mm[m] = ok ? mp[sl + 64u * (m + 1u)] : 0ull;
+        }
+      };
       const auto ticket = [=]() -> uint32_t { return (lane == 0) ? atomicAdd(sWork, 1u) : 0u; };
       const auto uniform = [](uint32_t v) { return (uint32_t) __builtin_amdgcn_readfirstlane((int) v); };
       uint32_t cur = itemOf(uniform(ticket()));
       uint32_t nxt = itemOf(uniform(ticket()));
-      // the lane's first prime of the current item and its further ones (k + 64 m:
-      // one more in tier 2, kT3 - 1 more in tier 3)
-      uint32_t k = primeOf(cur);
-      uint32_t p = (k != 0xffffffffu) ? primes[k] : 0u;
-      uint64_t M = (k != 0xffffffffu) ? magics[k] : 0ull;
+      // the lane's first prime of the current item and its further ones
+      // (slot + 64 m: one more in tier 2, kT3 - 1 more in tier 3)
+      uint32_t p;
+      uint64_t M;
       uint32_t pX[kT3 - 1];
       uint64_t MX[kT3 - 1];
-#pragma unroll
-      for (uint32_t m = 0; m < kT3 - 1; m++) {
-        const uint32_t km = k + 64u * (m + 1u);
-        const bool ok = k != 0xffffffffu && (cur >= u2 ? km < t3 : (cur >= u1 && m == 0 && km < t2));
-        pX[m] = ok ? primes[km] : 0u;
-        MX[m] = ok ? magics[km] : 0ull;
-      }
+      load_item(cur, p, M, pX, MX);
       // the bucket entries prefetched before tier 0 (arrived by now), then
       // the first batch in flight; a lane loads min(index, last) so that its
       // loads stay inside the list.  A batch that lies inside the list for
@@ -890,18 +906,11 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
           bW += 8u * nthreads;
           if (bE < bE1) load_batch(bq);
         }
-        const uint32_t kN = primeOf(nxt);
-        const uint32_t pN = (kN != 0xffffffffu) ? primes[kN] : 0u;
-        const uint64_t MN = (kN != 0xffffffffu) ? magics[kN] : 0ull;
+        uint32_t pN;
+        uint64_t MN;
         uint32_t pNX[kT3 - 1];
         uint64_t MNX[kT3 - 1];
-#pragma unroll
-        for (uint32_t m = 0; m < kT3 - 1; m++) {
-          const uint32_t km = kN + 64u * (m + 1u);
-          const bool ok = kN != 0xffffffffu && (nxt >= u2 ? km < t3 : (nxt >= u1 && m == 0 && km < t2));
-          pNX[m] = ok ? primes[km] : 0u;
-          MNX[m] = ok ? magics[km] : 0ull;
-        }
+        load_item(nxt, pN, MN, pNX, MNX);
         if (p) {
           if (cur < u1) {
             const uint32_t R = mod_magic31(B, p, M);             // p <= S < 2^31
