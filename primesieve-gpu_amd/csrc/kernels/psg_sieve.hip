@@ -84,13 +84,14 @@ constexpr uint32_t kCoprime30 = 0x208A2882u;
 // slots per row of the cofactor-ordered class table (8 coprime residues, repeated)
 constexpr uint32_t kCtabqSlots = 16;
 
-// R = B mod p for B < 2^63 using M = ceil(2^(64+L)/p):
-// floor(B/p) = mulhi(B, M) >> L.  Only the low 32 bits of B - q*p matter.
+// L = floor(log2 p) of a division magic M = ceil(2^(64+L)/p) (0 for p = 0)
 __device__ __forceinline__ uint32_t magic_shift(uint32_t p)
 {
   return 31u - (uint32_t) __clz((int) (p | 1u));
 }
 
+// R = B mod p for B < 2^63: floor(B/p) = mulhi(B, M) >> L.  Only the low 32
+// bits of B - q*p matter.
 __device__ __forceinline__ uint32_t mod_magic(uint64_t B, uint32_t p, uint64_t M)
 {
   const uint64_t q = __umul64hi(B, M) >> magic_shift(p);
@@ -110,15 +111,6 @@ __device__ __forceinline__ uint32_t mod_magic31(uint64_t B, uint32_t p, uint64_t
   const uint64_t H = (uint64_t) Bh * Mh + __umulhi(Bh, Ml) + __umulhi(Bl, Mh);
   const uint32_t r = Bl - (uint32_t) (H >> magic_shift(p)) * p;
   return min(r, r - p);
-}
-
-// floor(y / p) for y < 2^32 with the same magic: the high 64 bits of y * M
-// (y * M_hi plus the carry word of y * M_lo), shifted by L.
-__device__ __forceinline__ uint32_t div_magic32(uint32_t y, uint32_t p, uint64_t M)
-{
-  const uint32_t L = magic_shift(p);
-  const uint64_t hi = (uint64_t) y * (uint32_t) (M >> 32) + __umulhi(y, (uint32_t) M);
-  return (uint32_t) (hi >> 32) >> L;
 }
 
 __device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r)
@@ -402,7 +394,12 @@ __device__ __forceinline__ void strike_cands_lane(const uint32_t (&p)[kN], const
   for (uint32_t m = 0; m < kN; m++) {
     a[m] = p[m] / 30u;
     const uint32_t b = p[m] - a[m] * 30u;
-    const uint32_t t = div_magic32(30u * R[m] + 6u + p[m], p[m], M[m]);  // ceil((30R + 7) / p)
+    // t = ceil((30R + 7) / p) = floor((30R + 7 + p - 1) / p) from the magic's
+    // high word Mh = floor(2^(32+L) / p): floor(y Mh / 2^(32+L)) for
+    // y = 30R + 7 + p (one more than the dividend) is exact for y < 2^26 and
+    // p < 2^21, since y Mh / 2^(32+L) falls short of y / p by less than 1/p
+    // (and by more than 0: p is odd).
+    const uint32_t t = __umulhi(30u * R[m] + 7u + p[m], (uint32_t) (M[m] >> 32)) >> magic_shift(p[m]);
     const uint64_t* row = (const uint64_t*) ctabq + ((b >> 1) * kCtabqSlots + __popc(kCoprime30 & ((1u << t) - 1u)));
 #pragma unroll
     for (uint32_t k = 0; k < kK; k++) e[m][k] = row[k];

@@ -176,3 +176,32 @@ def test_four_and_five_candidate_bounds_are_tight():
             if found:
                 break
         assert found, k
+
+
+def _magic(p):
+    """The device's division magic of p (psg_prime_magics): ceil(2^(64+L) / p)."""
+    L = p.bit_length() - 1
+    return -(-(1 << (64 + L)) // p), L
+
+
+def test_first_cofactor_from_the_magic_high_word():
+    """The kernel takes t = ceil((30R + 7) / p) as floor(y Mh / 2^(32+L)) with
+    y = 30R + 7 + p and Mh the magic's high 32 bits: exact for tier-3 primes
+    below 30 * 2^16, checked at random remainders and at the remainders where
+    30R + 7 or 30R + 6 is a multiple of p (the cases a truncated magic would
+    get wrong)."""
+    rng = random.Random(19)
+    primes = [_random_prime(rng, 1 << 17, 30 << 16) for _ in range(300)]
+    primes += [131071, 1966079] + [x for x in range((30 << 16) - 200, 30 << 16) if _is_prime(x)]
+    for p in primes:
+        M, L = _magic(p)
+        Mh = M >> 32
+        assert Mh == (1 << (32 + L)) // p
+        Rs = [0, 1, p - 1, p - 2] + [rng.randrange(p) for _ in range(40)]
+        inv30 = pow(30, -1, p)
+        Rs += [(-7 * inv30) % p, (-6 * inv30) % p]
+        for R in Rs:
+            y = 30 * R + 7 + p
+            assert y < 1 << 26
+            t = ((y * Mh) >> 32) >> L
+            assert t == -(-(30 * R + 7) // p), (p, R)
