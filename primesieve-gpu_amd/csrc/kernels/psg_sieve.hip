@@ -824,6 +824,9 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
       const uint32_t pTop = (t3 > t2) ? primes[t3 - 1u] : 0u;
       const bool near = B < (uint64_t) pTop;
       const bool small = pTop < (30u << 16);
+      // the candidate bands: 6p > 30S (two classes), 8p > 30S (three), as
+      // p > floor(30S / 6) and p > floor(30S / 8) for integers p
+      const uint32_t cand2 = 5u * S, cand3 = (30u * S) >> 3;
       const uint64_t rmix = (((uint64_t) (u3 - u2) << 32) + u3 - 1u) / max(u3, 1u);
       const auto itemOf = [=](uint32_t t) -> uint32_t {
         if (t >= u3) return t;
@@ -947,8 +950,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
             }
             // lane 0 holds the item's smallest prime
             if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctab);
-            else if (small && 6ull * uniform(p) > 30ull * S) strike_cands_lane<kT3, 2>(pa, Ra, Ma, S, ctabq);
-            else if (small && 8ull * uniform(p) > 30ull * S) strike_cands_lane<kT3, 3>(pa, Ra, Ma, S, ctabq);
+            else if (small && uniform(p) > cand2) strike_cands_lane<kT3, 2>(pa, Ra, Ma, S, ctabq);
+            else if (small && uniform(p) > cand3) strike_cands_lane<kT3, 3>(pa, Ra, Ma, S, ctabq);
             else if (small) strike_big_lane<false, true, kT3>(pa, Ra, B, S, ctab);
             else strike_big_lane<false, false, kT3>(pa, Ra, B, S, ctab);
           }
@@ -1210,23 +1213,26 @@ __device__ __forceinline__ WindowBase window_base(const PsgWindow& w)
   return {uniform_f64(to_double(w.base)), uniform_f64(to_double(30ull * w.base + 7u))};
 }
 
-// emit(i, windowSegment, byte * 8 + bit) for hits i = 0, 1, ..., with byte
-// and bit within the window (so the window segment is the hit >> (segShift +
-// 3)); returns the number of hits of this stage.
+// emit(slot, byte * 8 + bit) for the stage's hits in order, with byte and
+// bit within the window (so the window segment is the hit >> (segShift + 3))
+// and slot the hit's LDS stage word: a thread's slots are PSG_BUCKET_THREADS
+// words apart, from slot0.  Returns the number of hits of this stage.
 template <typename Emit>
 __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, const PsgWindow& w,
                                                       const WindowBase& wb, BucketWalk& s,
                                                       uint32_t stride, const uint32_t* ctab,
-                                                      uint32_t limit, Emit emit)
+                                                      lds_u32* slot0, uint32_t limit, Emit emit)
 {
+  constexpr uint32_t kStep = PSG_BUCKET_THREADS;
   const auto* __restrict__ primes = gptr<uint32_t>(a.primes);
   const uint32_t bytes = w.segs << a.segShift;
   const double baseF = wb.baseF;
   const uint32_t V = 30u * bytes;                 // values covered by the window (< 2^31)
   const uint64_t v0 = 30ull * w.base + 7u;        // its first value (< 2^64 - 8)
   const double v0F = wb.v0F;
-  uint32_t n = 0;
-  while (n < limit) {
+  lds_u32* slot = slot0;
+  lds_u32* const slotEnd = slot0 + limit * kStep;
+  while (slot < slotEnd) {
     // The lanes of a wave take their next primes together, once every active
     // lane has finished its current one: the set-up (remainder, first
     // multiple or class start) then runs once per prime for the whole wave
@@ -1268,11 +1274,12 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
     }
     if (s.p == 0u) continue;                  // idle until the wave's next primes
     if (s.j == 8u) {
-      while (n < limit && s.off < V) {
+      while (slot < slotEnd && s.off < V) {
         if (((kCoprime30 >> s.qm) & 1u) && s.off != s.self) {
           const uint32_t byte = s.off / 30u;
           const uint32_t bit = bit_of_residue(s.off - 30u * byte);
-          emit(n++, byte >> a.segShift, (byte << 3) | bit);
+          emit(slot, (byte << 3) | bit);
+          slot += kStep;
         }
         s.qm = (s.qm == 29u) ? 0u : s.qm + 1u;
         s.off = (V - s.off > s.p) ? s.off + s.p : V;       // no 32-bit overflow
@@ -1280,11 +1287,15 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
       if (s.off >= V) s.p = 0u;
     } else {
       if (s.pos < bytes) {
-        // one exit test per hit: stop at the window's end or when the stage is full
-        const uint32_t room = limit - n;
+        // one exit test per hit: stop at the window's end or when the stage is
+        // full.  The loop steps the staged value byte * 8 + class by 8p
+        // (dense primes have 8p <= 30 * bytes < 2^31; bytes <= 2^25).
+        const uint32_t room = (uint32_t) (slotEnd - slot) / kStep;
         const uint32_t stop = ((uint64_t) room * s.p >= bytes - s.pos) ? bytes : s.pos + room * s.p;
-        for (; s.pos < stop; s.pos += s.p)
-          emit(n++, s.pos >> a.segShift, (s.pos << 3) | s.j);
+        uint32_t v = (s.pos << 3) | s.j;
+        const uint32_t v1 = stop << 3, p8 = s.p << 3;
+        for (; v < v1; v += p8, slot += kStep) emit(slot, v);
+        s.pos = v >> 3;
       }
       if (s.pos >= bytes) {
         if (++s.j == 8u) s.p = 0u;
@@ -1292,7 +1303,7 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
       }
     }
   }
-  return n;
+  return (uint32_t) (slot - slot0) / kStep;
 }
 
 // The window of fill work item `item` and the item's chunk in it.
@@ -1490,6 +1501,10 @@ psg_bucket_fill(uint64_t argsAddr)
     // C = 2^cs histogram copies per segment (uniform over the item)
     const uint32_t cs = (W <= PSG_BUCKET_HIST_WORDS / 4u) ? 2u : 1u;
     const uint32_t copy = tid & ((1u << cs) - 1u);
+    // this thread's histogram copy as an LDS byte address; segment s's
+    // counter is at histc + (s << cs2)
+    const uint32_t histc = (uint32_t) (uintptr_t) (lds_u32*) (hist + copy);
+    const uint32_t cs2 = cs + 2u;
     const uint32_t HW = W << cs;
     for (uint32_t i = tid; i < HW; i += nthreads) hist[i] = 0;
 
@@ -1500,10 +1515,13 @@ psg_bucket_fill(uint64_t argsAddr)
     const WindowBase wb = window_base(w);
     __syncthreads();
     for (;;) {
-      const uint32_t n = bucket_walk_stage(a, w, wb, st, stride, ctab, kStageF,
-          [&](uint32_t i, uint32_t seg, uint32_t hit) {
-            stage[i * nthreads + tid] = hit;
-            atomicAdd(&hist[(seg << cs) | copy], 1u);
+      const uint32_t n = bucket_walk_stage(a, w, wb, st, stride, ctab, (lds_u32*) (stage + tid), kStageF,
+          [&](lds_u32* slot, uint32_t hit) {
+            *slot = hit;
+            // the hit's window segment (< 256) and its counter in this copy
+            const uint32_t seg = (hit >> eshift) & (PSG_MAX_WINDOW_SEGS - 1u);
+            __hip_atomic_fetch_add((lds_u32*) (histc + (seg << cs2)), 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
           });
       const int more = __syncthreads_or((st.p != 0u || walk_peek(st) != 0u) ? 1 : 0);
 
