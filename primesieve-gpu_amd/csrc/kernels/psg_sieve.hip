@@ -716,9 +716,13 @@ __device__ __forceinline__ void fill_segment(const PsgSieveArgs& args, uint64_t 
 
 // Step 2 for segment `seg` at B of `run`: strike every sieving prime and the
 // segment's bucket list.  The queue counter must be 0; no barrier.
+// take() (called by thread 0 after tier 0) claims the workgroup's next
+// segment: its ticket and run lookup then overlap the other waves' strikes
+// instead of holding the whole workgroup at the barrier after them.
+template <typename Take>
 __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const PsgRun& run,
                                                uint32_t seg, uint64_t B, uint32_t* sieve,
-                                               const uint32_t* ctab, uint32_t* wsum)
+                                               const uint32_t* ctab, uint32_t* wsum, Take take)
 {
   const uint2* ctabq = (const uint2*) (ctab + kCtabWords + kVarWords + 3 * kTier0Slots);
   const uint32_t tid = threadIdx.x;
@@ -795,6 +799,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
         }
       }
     }
+
+    if (tid == 0) take();
 
     // tiers 1-3: a queue of work items shared by the waves of the workgroup
     // (an LDS counter), so that no wave is left with a long tail of work
@@ -1306,13 +1312,17 @@ __device__ __forceinline__ uint32_t bucket_walk_stage(const PsgBucketArgs& a, co
   return (uint32_t) (slot - slot0) / kStep;
 }
 
-// The window of fill work item `item` and the item's chunk in it.
-__device__ __forceinline__ PsgWindow window_of_item(const PsgBucketArgs& a, uint32_t item, uint32_t& chunk)
+// The window of fill work item `item`: wi on entry is the window of the
+// workgroup's previous item (w, or ~0 before the first), and items only grow,
+// so the window is w itself or is searched for after it.
+__device__ __forceinline__ void window_of_item(const PsgBucketArgs& a, uint32_t item, uint32_t& wi,
+                                               PsgWindow& w)
 {
+  if (wi != 0xffffffffu && item < w.wgBase + w.nChunks) return;
   const auto* win = gptr<PsgWindow>(a.windows);
-  const uint32_t w = last_le(a.nWindows, item, [&](uint32_t i) { return win[i].wgBase; });
-  chunk = item - win[w].wgBase;
-  return win[w];
+  const uint32_t lo = wi + 1u;                     // 0 before the first item
+  wi = lo + last_le(a.nWindows - lo, item, [&](uint32_t i) { return win[lo + i].wgBase; });
+  w = win[wi];
 }
 
 }  // namespace
@@ -1406,8 +1416,7 @@ psg_sieve_count(uint64_t argsAddr)
       fill_segment(args, B, sieve, small);
       __syncthreads();
     }
-    strike_segment(args, run, seg, B, sieve, ctab, wsum);
-    if (tid == 0) take();
+    strike_segment(args, run, seg, B, sieve, ctab, wsum, take);   // thread 0 takes the next one
     __syncthreads();                               // strikes done, next segment known
     const uint32_t next = sSeg;
     const bool hasNext = next < args.nSegs;
@@ -1486,6 +1495,8 @@ psg_bucket_fill(uint64_t argsAddr)
   const uint32_t eshift = a.segShift + 3u;             // window bit index -> window segment
   const uint32_t segBits = (1u << eshift) - 1u;        // window bit index -> segment bit index
   uint64_t appended = 0;
+  uint32_t wi = 0xffffffffu;                           // window of the previous item
+  PsgWindow w{};
   for (;;) {
     // The next item.  Every thread has read the previous item's number before
     // thread 0 overwrites it: the item's stages end with a barrier.
@@ -1493,8 +1504,8 @@ psg_bucket_fill(uint64_t argsAddr)
     __syncthreads();
     const uint32_t item = sItem;
     if (item >= a.nItems) break;
-    uint32_t c;
-    const PsgWindow w = window_of_item(a, item, c);
+    window_of_item(a, item, wi, w);
+    const uint32_t c = item - w.wgBase;
     const uint32_t W = w.segs;                           // <= PSG_MAX_WINDOW_SEGS <= blockDim
     const auto* __restrict__ listOff = gptr<uint32_t>(a.listOff) + w.firstSeg;
     uint32_t* listCount = (uint32_t*) a.listCount + w.firstSeg;
