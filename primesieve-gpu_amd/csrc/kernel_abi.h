@@ -21,8 +21,9 @@
 
 #define PSG_NCOUNTS 6          /* primes, twins, triplets, ..., sextuplets */
 #define PSG_MAX_SEG_BYTES (160u * 1024u)
-#define PSG_SIEVE_LDS_TAIL 3776u /* LDS after the segment: class table, counters, wave totals,
-                                    tier-0 primes and magics, cofactor-ordered class table */
+#define PSG_SIEVE_LDS_TAIL 3800u /* LDS after the segment: class table, counters, wave totals,
+                                    tier-0 primes and magics, class-row table,
+                                    cofactor-ordered class table */
 /* Pre-sieve tiles: the bit pattern of each group of small primes repeats
  * every P = product-of-the-group bytes.  A tile holds the P sliding 32-bit
  * windows of its pattern (window i = pattern bytes i..i+3, wrapping; a set

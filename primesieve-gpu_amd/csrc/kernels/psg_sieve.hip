@@ -81,8 +81,14 @@ __device__ __constant__ uint32_t kOffsets[8] = {7, 11, 13, 17, 19, 23, 29, 31};
 // the residues coprime to 30, ascending, and as a bit set (bit q for q = 0..29)
 __device__ __constant__ uint32_t kCoprimes[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 constexpr uint32_t kCoprime30 = 0x208A2882u;
-// slots per row of the cofactor-ordered class table (8 coprime residues, repeated)
-constexpr uint32_t kCtabqSlots = 16;
+// Slots per row of the cofactor-ordered class table: the 8 coprime residues,
+// then the first 5 again.  13 slots (26 words) put the rows of the 8 residues
+// used at 8 different LDS banks (26 (b >> 1) mod 32), so lanes reading rows of
+// different residues rarely meet in one bank.
+constexpr uint32_t kCtabqSlots = 13;
+// Words per row of the class-row table (8 entries, 4 words of padding): rows
+// indexed by the residue's rank start at 8 different banks (12 r mod 32).
+constexpr uint32_t kCtabrStride = 12;
 
 // L = floor(log2 p) of a division magic M = ceil(2^(64+L)/p) (0 for p = 0)
 __device__ __forceinline__ uint32_t magic_shift(uint32_t p)
@@ -226,11 +232,13 @@ __device__ __forceinline__ void lds_or(lds_u32* w, uint32_t mask)
   __hip_atomic_fetch_or(w, mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// The 8 class entries of p's residue b = p mod 30: two 16-byte LDS reads.
+// The 8 class entries of p's residue b = p mod 30: two 16-byte LDS reads
+// from the class-row table (the same entries as the class table, row r =
+// rank of b among the coprime residues, kCtabrStride words apart).
 struct ClassRow { uint32_t e[8]; };
-__device__ __forceinline__ ClassRow class_row(const uint32_t* ctab, uint32_t b)
+__device__ __forceinline__ ClassRow class_row(const uint32_t* ctabr, uint32_t b)
 {
-  const uint4* row = (const uint4*) &ctab[b * 8u];
+  const uint4* row = (const uint4*) &ctabr[__popc(kCoprime30 & ((1u << b) - 1u)) * kCtabrStride];
   const uint4 r0 = row[0], r1 = row[1];
   return ClassRow{{ r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w }};
 }
@@ -474,28 +482,38 @@ __device__ uint64_t mask_edges(uint64_t x, uint64_t gw, const PsgRun& r)
 }
 
 
-// Residue-class table, derived from first principles: for b = p mod 30 and
-// class j (offset o_j), q_j = o_j * b^-1 (mod 30) and e_j = (b*q_j - o_j)/30,
-// packed as q_j | (e_j + 1) << 16 (e_j is -1 only for o_j = 31, b = 1).
+// Residue-class table entry, derived from first principles: for b = p mod 30
+// and class j (offset o_j), q_j = o_j * b^-1 (mod 30) and
+// e_j = (b*q_j - o_j)/30, packed as q_j | (e_j + 1) << 16 (e_j is -1 only
+// for o_j = 31, b = 1); 0 for b not coprime to 30.
+__device__ __forceinline__ uint32_t class_entry(uint32_t b, uint32_t j)
+{
+  const uint32_t o = kOffsets[j];
+  if ((b % 2u) && (b % 3u) && (b % 5u)) {
+    for (uint32_t q = 1; q < 30; q++)
+      if ((b * q) % 30u == o % 30u) {
+        const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
+        return q | ((uint32_t) (e + 1) << 16);
+      }
+  }
+  return 0u;
+}
+
 __device__ __forceinline__ void init_ctab(uint32_t* ctab)
 {
-  for (uint32_t t = threadIdx.x; t < 240; t += blockDim.x) {
-    const uint32_t b = t >> 3, j = t & 7u;
-    const uint32_t o = kOffsets[j];
-    uint32_t ent = 0;
-    if ((b % 2u) && (b % 3u) && (b % 5u)) {
-      for (uint32_t q = 1; q < 30; q++)
-        if ((b * q) % 30u == o % 30u) {
-          const int e = ((int) (b * q) - (int) o) / 30;   // exact, may be -1
-          ent = q | ((uint32_t) (e + 1) << 16);
-          break;
-        }
-    }
-    ctab[t] = ent;
+  for (uint32_t t = threadIdx.x; t < 240; t += blockDim.x) ctab[t] = class_entry(t >> 3, t & 7u);
+}
+
+// The class-row table: row r holds the 8 entries of the r-th coprime residue.
+__device__ __forceinline__ void init_ctabr(uint32_t* ctabr)
+{
+  for (uint32_t t = threadIdx.x; t < 8u * kCtabrStride; t += blockDim.x) {
+    const uint32_t r = t / kCtabrStride, j = t % kCtabrStride;
+    ctabr[t] = (j < 8u) ? class_entry(kCoprimes[r], j) : 0u;
   }
 }
 
-// The same entries in cofactor order: ctabq[(b >> 1) * 16 + s] = (the entry
+// The same entries in cofactor order: ctabq[(b >> 1) * 13 + s] = (the entry
 // of the class whose cofactor residue is the (s mod 8)-th coprime residue,
 // the bit of that class) for b = p mod 30 coprime to 30 (one row per odd b;
 // the rows of odd b sharing a factor with 30 are zero).
@@ -506,7 +524,7 @@ __device__ __forceinline__ void init_ctab(uint32_t* ctab)
 __device__ __forceinline__ void init_ctabq(uint2* ctabq)
 {
   for (uint32_t t = threadIdx.x; t < 15u * kCtabqSlots; t += blockDim.x) {
-    const uint32_t b = 2u * (t / kCtabqSlots) + 1u, q = kCoprimes[t & 7u];
+    const uint32_t b = 2u * (t / kCtabqSlots) + 1u, q = kCoprimes[(t % kCtabqSlots) & 7u];
     uint2 e = make_uint2(0u, 0u);
     for (uint32_t j = 0; j < 8 && ((kCoprime30 >> b) & 1u); j++) {
       const uint32_t o = kOffsets[j];
@@ -594,12 +612,14 @@ constexpr uint32_t kT3 = 4;
 // LDS after the segment: the class table, the workgroup's counters (the six
 // sums, segment and run numbers, the queue counter, the list mode's wave
 // totals), the tier-0 primes and their magics (read once per workgroup),
-// then the cofactor-ordered class table (15 rows of 16 uint2).
+// then the class-row table (8 rows of kCtabrStride words) and the
+// cofactor-ordered class table (15 rows of kCtabqSlots uint2).
 constexpr uint32_t kCtabWords = 240;
 constexpr uint32_t kVarWords = 32;
 constexpr uint32_t kTier0Slots = PSG_TIER0_SLOTS;
 constexpr uint32_t kCtabqWords = 2u * 15u * kCtabqSlots;
-static_assert((kCtabWords + kVarWords + 3 * kTier0Slots + kCtabqWords) * 4 == PSG_SIEVE_LDS_TAIL,
+constexpr uint32_t kCtabrWords = 8u * kCtabrStride;
+static_assert((kCtabWords + kVarWords + 3 * kTier0Slots + kCtabqWords + kCtabrWords) * 4 == PSG_SIEVE_LDS_TAIL,
               "LDS tail layout");
 
 // The primes of work item `it` of a segment's queue (see strike_segment):
@@ -724,7 +744,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
                                                uint32_t seg, uint64_t B, uint32_t* sieve,
                                                const uint32_t* ctab, uint32_t* wsum, Take take)
 {
-  const uint2* ctabq = (const uint2*) (ctab + kCtabWords + kVarWords + 3 * kTier0Slots);
+  const uint32_t* ctabr = ctab + kCtabWords + kVarWords + 3 * kTier0Slots;     // 16-byte aligned
+  const uint2* ctabq = (const uint2*) (ctabr + kCtabrWords);
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
   const uint32_t lane = tid & 63u;
@@ -931,8 +952,8 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
             const uint32_t R2 = p2 ? mod_magic31(B, p2, MX[0]) : 0u;
             const uint32_t pMin = uniform(p), pMin2 = uniform(p2);
             const uint32_t a = p / 30u, a2 = p2 / 30u;
-            const ClassRow row = class_row(ctab, p - a * 30u);
-            const ClassRow row2 = class_row(ctab, p2 - a2 * 30u);
+            const ClassRow row = class_row(ctabr, p - a * 30u);
+            const ClassRow row2 = class_row(ctabr, p2 - a2 * 30u);
             strike_tier2(p, a, R, B, S, row, pMin, nearSmall);
             if (p2) strike_tier2(p2, a2, R2, B, S, row2, pMin2, nearSmall);
           } else {
@@ -955,11 +976,11 @@ __device__ __forceinline__ void strike_segment(const PsgSieveArgs& args, const P
               for (uint32_t m = 0; m < kT3; m++) Ra[m] = mod_magic(B, pa[m], Ma[m]);
             }
             // lane 0 holds the item's smallest prime
-            if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctab);
+            if (near) strike_big_lane<true, false, kT3>(pa, Ra, B, S, ctabr);
             else if (small && uniform(p) > cand2) strike_cands_lane<kT3, 2>(pa, Ra, Ma, S, ctabq);
             else if (small && uniform(p) > cand3) strike_cands_lane<kT3, 3>(pa, Ra, Ma, S, ctabq);
-            else if (small) strike_big_lane<false, true, kT3>(pa, Ra, B, S, ctab);
-            else strike_big_lane<false, false, kT3>(pa, Ra, B, S, ctab);
+            else if (small) strike_big_lane<false, true, kT3>(pa, Ra, B, S, ctabr);
+            else strike_big_lane<false, false, kT3>(pa, Ra, B, S, ctabr);
           }
         }
         cur = nxt;
@@ -1377,7 +1398,8 @@ psg_sieve_count(uint64_t argsAddr)
 
   const auto* runs = gptr<PsgRun>(args.runs);
   init_ctab(ctab);
-  init_ctabq((uint2*) (ctab + kCtabWords + kVarWords + 3 * kTier0Slots));
+  init_ctabr(ctab + kCtabWords + kVarWords + 3 * kTier0Slots);
+  init_ctabq((uint2*) (ctab + kCtabWords + kVarWords + 3 * kTier0Slots + kCtabrWords));
   {
     uint32_t* t0p = ctab + kCtabWords + kVarWords;
     uint64_t* t0m = (uint64_t*) (t0p + kTier0Slots);
