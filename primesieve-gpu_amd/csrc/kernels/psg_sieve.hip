@@ -1505,7 +1505,9 @@ psg_bucket_fill(uint64_t argsAddr)
   uint32_t* stage = lds;                               // [kStageF][nthreads], then sorted
   // [W][C] counts per segment and copy (copy = thread mod C), then scatter cursors
   uint32_t* hist = stage + PSG_BUCKET_STAGE;
-  // [W] first sorted slot | fit << 16 | entry index of the segment's run << 32
+  // [W] end of the segment's written slots (first sorted slot + fit) | (the
+  // list index of its first slot minus that slot, mod 2^32) << 32: sorted slot
+  // i < end goes to list index base + i
   uint64_t* place = (uint64_t*) (hist + PSG_BUCKET_HIST_WORDS);
   __shared__ __align__(16) uint32_t ctab[240];
   __shared__ uint32_t waveTotal[PSG_BUCKET_THREADS / 64u];
@@ -1634,14 +1636,15 @@ psg_bucket_fill(uint64_t argsAddr)
         const uint32_t room = listOff[tid + 1u] - listOff[tid];
         if ((uint64_t) at + cnt > room) atomicOr((uint32_t*) a.overflow, 1u);
         const uint32_t fit = (at >= room) ? 0u : min(cnt, room - at);
-        place[tid] = (uint64_t) (f | (fit << 16)) | ((uint64_t) (listOff[tid] + at) << 32);
+        place[tid] = (uint64_t) (f + fit) | ((uint64_t) (listOff[tid] + at - f) << 32);
       }
       __syncthreads();
 
       // Write-out, eight slots at a time: their stage reads, then their
-      // segments' (place, destination) reads, then the stores.  Slots past
-      // `total` hold stale values; their segment index is masked into range
-      // and they store nothing.
+      // segments' placements, then the stores.  Slot i holds an entry of its
+      // segment's run (which starts at or before i), so it is written when
+      // it lies below the run's end.  Slots past `total` hold stale values;
+      // their segment index is masked into range and they store nothing.
 #pragma unroll
       for (uint32_t g = 0; g < kStageF; g += 8) {
         if (g * nthreads >= total) break;                // total is uniform
@@ -1654,9 +1657,8 @@ psg_bucket_fill(uint64_t argsAddr)
 #pragma unroll
         for (uint32_t q = 0; q < 8; q++) {
           const uint32_t i = (g + q) * nthreads + tid;
-          const uint32_t first = (uint32_t) pl[q], dst = (uint32_t) (pl[q] >> 32);
-          const uint32_t r = i - (first & 0xffffu);
-          if (i < total && r < (first >> 16)) entries[dst + r] = v[q] & segBits;
+          const uint32_t end = (uint32_t) pl[q], base = (uint32_t) (pl[q] >> 32);
+          if (i < min(total, end)) entries[base + i] = v[q] & segBits;
         }
       }
       appended += total;
