@@ -52,6 +52,28 @@ void DeviceBuffer::release()
   bytes_ = 0;
 }
 
+void* PinnedBuffer::ensure(const HipApi* api, size_t bytes)
+{
+  if (ptr_ && bytes_ >= bytes)
+    return ptr_;
+  const size_t want = std::max(bytes, 2 * bytes_);   // grow geometrically
+  release();
+  api_ = api;
+  void* p = nullptr;
+  hipCheck(api, api->HostMalloc(&p, want ? want : 1, 0), "hipHostMalloc");
+  ptr_ = p;
+  bytes_ = want;
+  return ptr_;
+}
+
+void PinnedBuffer::release()
+{
+  if (ptr_ && api_)
+    (void) api_->HostFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
 namespace {
 
 std::mutex g_tuningMutex;
@@ -96,6 +118,8 @@ HipBackend::~HipBackend()
   entries_[1].release();
   flags_.release();
   if (hostFlags_) (void) api_->HostFree(hostFlags_);
+  pinnedDesc_.release();
+  pinnedCounts_.release();
   for (hipEvent_t e : phaseEvents_) (void) api_->EventDestroy(e);
   if (evSetup_) (void) api_->EventDestroy(evSetup_);
   if (evStart_) (void) api_->EventDestroy(evStart_);
@@ -656,8 +680,17 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
   listCountAddr_ = listCount;
 
   // ---- the descriptor blob ----------------------------------------------------
-  staging_.assign(descBytes, 0);
-  unsigned char* blob = staging_.data();
+  unsigned char* blob;
+  if (opt.noAlloc)
+  {
+    staging_.assign(descBytes, 0);
+    blob = staging_.data();
+  }
+  else
+  {
+    blob = (unsigned char*) pinnedDesc_.ensure(api_, descBytes);
+    std::memset(blob, 0, descBytes);
+  }
   PsgClearArgs clr;
   std::memset(&clr, 0, sizeof clr);
   clr.addr[0] = counts;    clr.words[0] = (uint32_t) (countBytes / 4);
@@ -828,12 +861,17 @@ bool HipBackend::launchPlan(uint32_t outs, const std::vector<uint32_t>& room, co
     }
   }
   hipCheck(api_, api_->EventRecord(evStop_, stream_), "hipEventRecord");
-  hostCounts_.resize((size_t) outs * PSG_NCOUNTS);
+  const size_t nCounts = (size_t) outs * PSG_NCOUNTS;
+  hostCounts_.resize(nCounts);
+  uint64_t* countsTo = opt.noAlloc ? hostCounts_.data()
+                                   : (uint64_t*) pinnedCounts_.ensure(api_, nCounts * 8);
   hipCheck(api_, api_->MemcpyAsync(hostFlags_, (void*) (uintptr_t) flags, 16, hipMemcpyDeviceToHost, stream_),
            "hipMemcpyAsync flags");
-  hipCheck(api_, api_->MemcpyAsync(hostCounts_.data(), (void*) (uintptr_t) counts, hostCounts_.size() * 8,
+  hipCheck(api_, api_->MemcpyAsync(countsTo, (void*) (uintptr_t) counts, nCounts * 8,
                                    hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync counts");
   hipCheck(api_, api_->StreamSynchronize(stream_), "sieve kernels");
+  if (countsTo != hostCounts_.data())
+    std::copy(countsTo, countsTo + nCounts, hostCounts_.begin());
 
   float ms = 0.f;
   hipCheck(api_, api_->EventElapsedTime(&ms, evStart_, evStop_), "hipEventElapsedTime");

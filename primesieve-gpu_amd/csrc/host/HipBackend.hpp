@@ -56,6 +56,27 @@ private:
   size_t bytes_ = 0;
 };
 
+/// Pinned host memory (hipHostMalloc), grown on demand; contents are not kept
+/// when it grows.  Copies from and to it are asynchronous DMA, while a copy
+/// from pageable memory is staged by the runtime before hipMemcpyAsync
+/// returns (65 us for a call's descriptors).
+class PinnedBuffer
+{
+public:
+  PinnedBuffer() {}
+  ~PinnedBuffer() { release(); }
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+
+  void* ensure(const HipApi* api, size_t bytes);
+  void release();
+
+private:
+  const HipApi* api_ = nullptr;
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
 /// An interval [lo, hi] whose six counts are added to output slot `out`.
 struct Interval
 {
@@ -222,7 +243,9 @@ private:
   DeviceBuffer flags_;         // uint32 overflow flag, pad, uint64 entries appended
   uint64_t* hostFlags_ = nullptr;      // pinned copy of flags_
   std::vector<uint64_t> hostCounts_;
-  std::vector<unsigned char> staging_;
+  std::vector<unsigned char> staging_;  // descriptor blob of calls that must not allocate
+  PinnedBuffer pinnedDesc_;            // descriptor blob of the other calls
+  PinnedBuffer pinnedCounts_;          // their counts, copied back
   std::vector<hipEvent_t> phaseEvents_;
 
   // the plan of the current call (built by execute, launched by launchPlan)
