@@ -544,11 +544,19 @@ void HipBackend::execute(const std::vector<Interval>& iv, uint32_t outs, uint64_
     slabs_.push_back(cur);
   }
 
+  // the segments of a run share their expectation: one room per run
   std::vector<uint32_t> room(segExpect_.size());
+  double lastE = -1.0;
+  uint32_t lastRoom = 0;
   for (size_t s = 0; s < room.size(); s++)
   {
     const double E = segExpect_[s];
-    room[s] = E > 0 ? (uint32_t) std::min(std::ceil(E + bucketMargin_ * std::sqrt(E) + 64.0), 4.0e9) : 0u;
+    if (E != lastE)
+    {
+      lastE = E;
+      lastRoom = E > 0 ? (uint32_t) std::min(std::ceil(E + bucketMargin_ * std::sqrt(E) + 64.0), 4.0e9) : 0u;
+    }
+    room[s] = lastRoom;
   }
   if (!launchPlan(outs, room, opt))
   {
