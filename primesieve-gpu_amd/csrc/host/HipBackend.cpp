@@ -384,6 +384,25 @@ hipEvent_t HipBackend::phaseEvent(size_t i)
   return phaseEvents_[i];
 }
 
+bool HipBackend::tilesFitLds() const
+{
+  if (tilesLds_ >= 0)
+    return tilesLds_ == 1;
+  const uint32_t base = sieveBytes_ + PSG_SIEVE_LDS_TAIL;
+  const uint32_t with = base + 4u * PSG_SMALL_TILE_WORDS;
+  bool fit = with <= ldsBytes_;
+  if (fit && fnSieve_)
+  {
+    int occWith = 0, occWithout = 0;
+    if (api_->ModuleOccupancy(&occWith, fnSieve_, (int) wgSize_, with) == hipSuccess &&
+        api_->ModuleOccupancy(&occWithout, fnSieve_, (int) wgSize_, base) == hipSuccess)
+      fit = occWith >= occWithout;
+  }
+  if (fnSieve_)
+    tilesLds_ = fit ? 1 : 0;
+  return fit;
+}
+
 uint32_t HipBackend::persistentGrid(uint32_t nSegs)
 {
   if (!persistentWGs_)

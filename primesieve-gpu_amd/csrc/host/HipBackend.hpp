@@ -180,10 +180,10 @@ private:
                           const std::function<void()>& record);
   // LDS of the segment kernel: the segment, its tail and (when they fit) the
   // small pre-sieve tiles
-  bool tilesFitLds() const
-  {
-    return (uint64_t) sieveBytes_ + PSG_SIEVE_LDS_TAIL + 4ull * PSG_SMALL_TILE_WORDS <= ldsBytes_;
-  }
+  // The small pre-sieve tiles go into LDS when they fit beside the segment and
+  // do not lower the number of workgroups a CU holds (a 64 KiB segment of 8
+  // waves runs two workgroups per CU only with its tiles in global memory).
+  bool tilesFitLds() const;
   uint32_t sieveLds() const
   {
     return sieveBytes_ + PSG_SIEVE_LDS_TAIL
@@ -204,6 +204,7 @@ private:
   uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
   uint32_t windowSegs_ = 128;     // segments per bucket window of a single interval
   uint32_t persistentWGs_ = 0;    // resident workgroups of the sieve kernel
+  mutable int tilesLds_ = -1;     // tilesFitLds(), once the sieve kernel is loaded
   uint32_t fillWGs_ = 0;          // resident workgroups of the bucket fill
   uint32_t bucketMargin_ = 6;     // bucket list room: expected + margin * sqrt(expected) + 64
 
