@@ -53,7 +53,7 @@ typedef struct psg_tuning {
   uint32_t work_group_size;   /* threads per workgroup: 64 * waves */
   uint32_t segments_per_launch;
   uint32_t slab_segments;     /* segments per big-prime bucket slab */
-  uint32_t bucket_factor;     /* primes > bucket_factor * sieve_bytes use buckets; 0 = auto (7) */
+  uint32_t bucket_factor;     /* primes > bucket_factor * sieve_bytes use buckets; 0 = auto (7, batches 12) */
   uint32_t presieve;          /* 1 = pre-sieve tiles for 7..47 */
   uint32_t bucket_margin;     /* room of a segment's bucket list beyond the expected
                                  count, in standard deviations; 0 = auto (6) */

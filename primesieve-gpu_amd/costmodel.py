@@ -37,7 +37,7 @@ DEFAULT = (131072, 16)          # the backend's own default geometry
 BUCKET_FACTORS = (4, 7, 12)
 GRID3 = [(s, w, b) for s in SEG_BYTES for w in WAVES
          for b in (BUCKET_FACTORS if s & (s - 1) == 0 else (0,))]
-DEFAULT3 = (131072, 16, 7)
+DEFAULT3 = (131072, 16, 12)   # the backend's default for a batch (12 segments; 7 for one call)
 
 
 def prime_count_estimate(x):

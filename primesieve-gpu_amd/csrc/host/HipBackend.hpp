@@ -114,6 +114,7 @@ struct ExecOptions
   bool pipeline = false;          // fill slab k+1 on a second stream while slab k is sieved
   bool graph = false;             // replay a captured launch series (captured once per shape)
   bool noAlloc = false;           // per-call device memory from psg_scratch only
+  uint32_t bucketFactor = 0;      // bucket threshold in segments for this call (0: the backend's)
   uint64_t listOut = 0;           // list mode: uint32_t* for the primes (one segment per interval)
   uint64_t listOffsets = 0;       // list mode: const uint32_t[outs], first output index per interval
 };
@@ -200,6 +201,7 @@ private:
   uint32_t wgSize_ = 0;
   uint32_t segmentsPerLaunch_ = 0;
   uint32_t bucketFactor_ = 8;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
+  bool bucketAuto_ = true;        // bucketFactor_ is the default, not set by a tuning
   uint32_t slabSegments_ = 0;     // segments per slab of a single interval
   uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
   uint32_t windowSegs_ = 128;     // segments per bucket window of a single interval

@@ -42,10 +42,11 @@ def poisson_workload(n, qps, seed, alpha=1.2, min_len=2 ** 12, max_len=2 ** 36):
     return arrivals, qs
 
 
-def bucket_entries_estimate(start, stop, seg_bytes=131072, factor=7):
+def bucket_entries_estimate(start, stop, seg_bytes=131072, factor=12):
     """Expected big-prime bucket entries of one query: 8 * bytes * sum 1/p over
     the bucket primes (p > factor * seg_bytes; the backend's default bucket
-    threshold is 7 segments), by Mertens' estimate."""
+    threshold for a batch, which is how the loop counts, is 12 segments), by
+    Mertens' estimate."""
     lo = float(factor * seg_bytes)
     hi = math.sqrt(max(stop, 4))
     if hi <= lo:
