@@ -203,7 +203,8 @@ private:
   uint32_t bucketFactor_ = 8;     // primes > bucketFactor_ * S use the bucket tier (0 = never)
   bool bucketAuto_ = true;        // bucketFactor_ is the default, not set by a tuning
   uint32_t slabSegments_ = 0;     // segments per slab of a single interval
-  uint32_t batchSlabSegs_ = 8192; // segments per slab of a batch
+  uint32_t batchSlabSegs_ = 32768; // segments per slab of a batch (8192: 9601, 16384: 9720,
+                                   // 32768: 9757 intervals/s on the 8192-interval batch)
   uint32_t windowSegs_ = 128;     // segments per bucket window of a single interval
   uint32_t persistentWGs_ = 0;    // resident workgroups of the sieve kernel
   mutable int tilesLds_ = -1;     // tilesFitLds(), once the sieve kernel is loaded
