@@ -12,18 +12,18 @@ Prints ONE JSON line:
   roofline      the segment kernel (psg_sieve_count) against the measured LDS
                 roofline: achieved = algorithmic LDS bytes per launch / average
                 launch duration (HIP events on the backend's stream), with
-                algorithmic LDS bytes = 8*C + 8*W (C = crossings by sieving
-                primes >= 53 -- 7..47 are pre-sieve tiles: one 4-byte read
-                + one 4-byte write each; W =
-                32-bit words of the bit array: one fill write + one count
-                read); peak = the same 8-bytes-per-ds_or_b32 figure (the
-                sieve's strike) measured by the conflict-free LDS
-                microbenchmark on the same device;
+                algorithmic LDS bytes = 8*C + 8*W, the measurement
+                contract's fixed convention for this path (C = crossings by
+                the sieving primes >= 29: one 4-byte read + one 4-byte write
+                each; W = 32-bit words of the bit array: one fill write +
+                one count read); peak = the same 8-bytes-per-ds_or_b32
+                figure (the sieve's strike) measured by the conflict-free
+                LDS microbenchmark on the same device;
                 traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE /
-                WRITE_SIZE summary in profiles/ when present; frac_c_over_p29
-                = the same fraction with C over p >= 29 (the pre-sieved
-                primes 29..47 counted as struck), the convention of the
-                measurement contract for this path
+                WRITE_SIZE summary in profiles/ when present;
+                frac_struck_p53 = the stricter fraction with C over the
+                primes this kernel strikes in LDS (p >= 53: 7..47 are
+                pre-sieve tiles, whose crossings cost no atomics)
   cpu_baseline  the reference CPU sieve (oracle/_ref, built from the
                 reference sources) on the host's CPU share (16 threads),
                 counting all six types over the same interval in one pass;
@@ -361,24 +361,29 @@ def main():
     got, dt, st, avg_launch_ms, launches = headline(m, args.steps, args.warmup)
     value = (STOP - START + 1) * args.steps / dt
     work = m.algorithmic_work(START, STOP, st.sieve_bytes)
-    lds_bytes_launch = (8 * work.crossings + 8 * work.words32) / max(1, launches)
-    crossings_p29 = work.crossings + tile_prime_crossings(max(START, 7), STOP)
-    achieved = lds_bytes_launch / (avg_launch_ms / 1e3) / 1e9
+    seconds = avg_launch_ms / 1e3
+    # C over the sieving primes >= 29 (the convention), and over the primes
+    # struck in LDS (>= 53; 29..47 are pre-sieve tiles)
+    crossings = work.crossings + tile_prime_crossings(max(START, 7), STOP)
+    lds_bytes_launch = (8 * crossings + 8 * work.words32) / max(1, launches)
+    struck_bytes_launch = (8 * work.crossings + 8 * work.words32) / max(1, launches)
+    achieved = lds_bytes_launch / seconds / 1e9
+    achieved_struck = struck_bytes_launch / seconds / 1e9
     lds = m.lds_roofline()
     peak = lds["or_conflict_free_gbs"]          # the sieve strikes with ds_or_b32
     roofline = {"bound": "lds", "achieved": achieved, "peak": peak, "unit": "GB/s",
                 "frac": achieved / peak, "traffic": profiled_traffic(),
                 "kernel": "psg_sieve_count", "avg_launch_ms": avg_launch_ms,
                 "launches_per_step": launches, "lds_bytes_per_launch": lds_bytes_launch,
-                "crossings": work.crossings, "words32": work.words32,
+                "crossings": crossings, "crossings_primes_from": 29, "words32": work.words32,
                 "peak_source": "measured ds_or_b32 microbenchmark (psg_lds_and mode 2, conflict-free)",
+                "crossings_struck_p53": work.crossings,
+                "lds_bytes_struck_per_launch": struck_bytes_launch,
+                "achieved_struck": achieved_struck,
+                "frac_struck_p53": achieved_struck / peak,
                 # the same strike with random word addresses, the access pattern of a scattered sieve
                 "peak_random_address": lds["or_random_gbs"],
-                "frac_vs_random_address_peak": achieved / lds["or_random_gbs"],
-                # C over p >= 29 (the tile primes 29..47 counted as if struck in LDS)
-                "crossings_p29": crossings_p29,
-                "frac_c_over_p29": (8 * crossings_p29 + 8 * work.words32) / max(1, launches)
-                                   / (avg_launch_ms / 1e3) / 1e9 / peak,
+                "frac_struck_vs_random_address_peak": achieved_struck / lds["or_random_gbs"],
                 "measured": lds}
     out = {
         "metric": "integers sieved/s on 1 MI355X, exact counts",
