@@ -341,7 +341,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the count path runs on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
+              "sharding an interval over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import psgpu as m
